@@ -1,0 +1,109 @@
+// device.hpp — per-point update formulas for every stencil (device code).
+//
+// Each formula keeps the association order of the reference's CPU loop
+// (or, where the reference has no CPU loop, of the first-time-step equations of
+// its generated kernel text), so that with FP contraction disabled the GPU
+// result equals a CPU evaluation of the same expression bit for bit.
+// OpenCL's rsqrt (approximate in the reference's kernels) is replaced by the
+// correctly rounded 1.0f/sqrtf(x), which is what the reference's CPU loop
+// computes (gradient-2d.cpp:697-699).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace ot {
+namespace dev {
+
+__device__ __forceinline__ float rsq(float x) { return 1.0f / __builtin_sqrtf(x); }
+__device__ __forceinline__ float sq(float x) { return x * x; }
+
+// jacobi-1d.cpp:633-635
+__device__ __forceinline__ float j1d3(float l, float c, float r) { return 0.333f * ((l + c) + r); }
+// jacobi-1d-5pt.cpp:507-511
+__device__ __forceinline__ float j1d5(float m2, float m1, float c, float p1, float p2) {
+  return 0.2f * ((((m2 + m1) + c) + p1) + p2);
+}
+// jacobi-1d-7pt.cpp:515-521 (0.2f kept as in the reference)
+__device__ __forceinline__ float j1d7(float m3, float m2, float m1, float c, float p1, float p2,
+                                      float p3) {
+  return 0.2f * ((((((m3 + m2) + m1) + c) + p1) + p2) + p3);
+}
+// jacobi-2d.cpp:824-828 (1.0f/5.0f == 0.2f): left, centre, right, top(y-1), bottom(y+1)
+__device__ __forceinline__ float j2d(float l, float c, float r, float t, float b) {
+  return 0.2f * ((((l + c) + r) + t) + b);
+}
+// poisson-2d.cpp:644-648
+__device__ __forceinline__ float p2d(float tl, float tc, float tr, float ml, float mc, float mr,
+                                     float bl, float bc, float br) {
+  return 8.0f * mc - (((((((tl + tc) + tr) + ml) + mr) + bl) + bc) + br);
+}
+// gradient-2d.cpp:691-699 (float path)
+__device__ __forceinline__ float g2d(float l, float c, float r, float t, float b) {
+  return c + rsq((((0.0001f + sq(c - l)) + sq(c - r)) + sq(c - t)) + sq(c - b));
+}
+// jacobi-3d.cpp:734-741: order (y-1, c, y+1, x-1, x+1, z+1, z-1)
+__device__ __forceinline__ float j3d(float xm, float xp, float ym, float yp, float zm, float zp,
+                                     float c) {
+  return 0.143f * ((((((ym + c) + yp) + xm) + xp) + zp) + zm);
+}
+// gradient-3d.cpp:305-307: val0..val6 = x-1, c, x+1, y-1, y+1, z-1, z+1
+__device__ __forceinline__ float g3d(float xm, float xp, float ym, float yp, float zm, float zp,
+                                     float c) {
+  return c + rsq((((((0.0001f + sq(c - xm)) + sq(c - xp)) + sq(c - ym)) + sq(c - yp)) + sq(c - zm)) +
+                 sq(c - zp));
+}
+// Poisson-Jacobi relaxation with right-hand side f and unit spacing:
+// u' = (sum6 - f) / 6 (no reference driver; the reference's poisson-3d has no RHS)
+__device__ __forceinline__ float p3d7(float xm, float xp, float ym, float yp, float zm, float zp,
+                                      float f) {
+  return ((((((xm + xp) + ym) + yp) + zm) + zp) - f) / 6.0f;
+}
+
+// cdsc-tv-update-2d.cpp:290-309. c3 duplicates c1 as in the reference (its
+// MATLAB comment repeats c1's formula); V and F read-only, alpha a parameter.
+__device__ __forceinline__ float tv2d(float tc, float tr, float ml, float mc, float mr, float bl,
+                                      float bc, float v, float f, float alpha) {
+  const float c1 = (mc * rsq((1e-8f + sq(bc - mc)) + sq(mr - mc))) / v;
+  const float c2 = (mc * rsq((1e-8f + sq(mc - tc)) + sq(tr - tc))) / v;
+  const float c3 = (mc * rsq((1e-8f + sq(bc - mc)) + sq(mr - mc))) / v;
+  const float c4 = (mc * rsq((1e-8f + sq(bl - ml)) + sq(mc - ml))) / v;
+  return (1.0f / ((((alpha + c1) + c2) + c3) + c4)) *
+         ((((alpha * f + c1 * bc) + c2 * tc) + c3 * mr) + c4 * ml);
+}
+
+// rician-denoise-2d.cpp:154-162, :271-281, :322-327.
+struct RicianK {
+  float dt, sigma2, gamma;
+};
+__device__ __forceinline__ float rician_g(float c, float l, float r, float t, float b) {
+  return rsq((((1.0e-20f + sq(c - r)) + sq(c - l)) + sq(c - b)) + sq(c - t));
+}
+__device__ __forceinline__ float rician_u(float c, float f, float ul, float ur, float ut, float ub,
+                                          float gl, float gr, float gt, float gb, RicianK k) {
+  float rr = (c * f) / k.sigma2;
+  rr = (rr * (2.38944f + rr * (0.950037f + rr))) / (4.65314f + rr * (2.57541f + rr * (1.48937f + rr)));
+  return (c + k.dt * ((((ub * gb + ut * gt) + ur * gr) + ul * gl) + (k.gamma * f) * rr)) /
+         (1.0f + k.dt * ((((gr + gl) + gb) + gt) + k.gamma));
+}
+
+// fdtd-2d.cpp:281-283, :290-292, :320-322
+__device__ __forceinline__ float fdtd_ey(float ey, float hz, float hz_top) { return ey - 0.5f * (hz - hz_top); }
+__device__ __forceinline__ float fdtd_ex(float ex, float hz, float hz_left) { return ex - 0.5f * (hz - hz_left); }
+__device__ __forceinline__ float fdtd_hz(float hz, float ex, float ex_right, float ey, float ey_bottom,
+                                         bool physical) {
+  return physical ? hz - 0.7f * (((ex_right - ex) + ey_bottom) - ey)
+                  : hz - 0.7f * (((ex_right - ex) + ey_bottom) + ey);
+}
+
+// Map a linear workgroup id to a logical tile id so that the workgroups the
+// hardware places on one XCD (ids congruent mod 8) get a contiguous run of
+// tiles, keeping shared halos within that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int xcd = b & 7, local = b >> 3;
+  const int q = n >> 3, rem = n & 7;
+  return xcd * q + (xcd < rem ? xcd : rem) + local;
+}
+
+}  // namespace dev
+}  // namespace ot

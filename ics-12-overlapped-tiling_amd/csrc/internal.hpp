@@ -1,0 +1,66 @@
+// internal.hpp — declarations shared by the engine's translation units.
+#pragma once
+#include "ot/ot.h"
+
+#include <stdint.h>
+
+namespace ot {
+
+constexpr int kMaxT = 8;          // largest time tile with compiled kernels
+constexpr int kMaxThreads = 1024;
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+// Record an error message for ot_last_error() and return the status.
+ot_status set_error(ot_status st, const char* fmt, ...);
+
+ot_status derive(const ot_params& p, ot_derived& d);
+
+// Fill threads_per_block / lds_bytes for the kernel family that runs p.
+ot_status kernel_shape(const ot_params& p, ot_derived& d);
+
+// Geometry of one launch, passed by value to every kernel.
+struct LaunchGeom {
+  int32_t n[3];          // domain extents
+  int32_t tile[3];       // output tile per workgroup
+  int32_t grid[3];       // workgroups per axis
+  int64_t sy, sz;        // strides (elements)
+  int64_t origin;        // element index of interior (0,0,0)
+  int32_t phase_limit;
+  int32_t variant;
+  float coef[4];
+};
+
+// Per-launch field pointers (up to 3 rw fields, 2 ro fields).
+struct FieldPtrs {
+  const float* src[3];
+  float* dst[3];
+  const float* ro[2];
+};
+
+// Launch one overlapped-tile kernel advancing t_eff steps. Implemented per
+// dimensionality in k1d.hip / k2d.hip / k3d.hip.
+// With prepare=true nothing is launched: the kernel instantiation is looked up
+// and its dynamic-LDS limit raised when it needs more than 64 KiB, so that a
+// later launch (possibly inside a hipGraph capture) needs no attribute call.
+ot_status launch_1d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
+                    void* stream, bool prepare);
+ot_status launch_2d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
+                    void* stream, bool prepare);
+ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
+                    void* stream, bool prepare);
+
+// Per-stencil 3D launchers (k3d_<name>.hip).
+ot_status launch_3d_j3d(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3d_g3d(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3d_p3d7(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3d_p3d27(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+
+// Raise the dynamic-LDS limit of a kernel that needs more than 64 KiB.
+ot_status prepare_lds(const void* kernel, int lds_bytes);
+
+// Launch shapes (host-side, also used by ot_derive).
+void shape_1d(int stencil, int T, int ept, const int32_t tile[3], int* threads, int* lds);
+void shape_2d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* by, int* lds);
+void shape_3d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* by, int* lds);
+
+}  // namespace ot

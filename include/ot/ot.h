@@ -1,0 +1,216 @@
+/*
+ * ot.h — C ABI of the MI355X overlapped time-tiling stencil engine.
+ *
+ * The reference (jholewinski/ics-12-overlapped-tiling) exposes this path in
+ * three layers: an in-process generator API (GeneratorParams +
+ * XGenerator::generate, opencl/jacobi-2d/jacobi-2d.cpp:21-173,
+ * opencl/common/inc/ProgramGenerator.hpp:10-69), the generated kernel ABI
+ * (kernel_func(input, [aux...], output, baseTime), e.g.
+ * opencl/jacobi-2d/jacobi-2d.cpp:184-190) driven by a host time loop
+ * (opencl/jacobi-2d/jacobi-2d.cpp:888-933), and a driver CLI whose stdout
+ * "Key: value" lines feed the sweep scripts (scripts/utils.py:104-155).
+ * This header replaces the first two layers; the ot-<stencil> drivers built on
+ * top of it keep the third.
+ *
+ * Plain C: no exceptions cross this boundary, every entry point returns an
+ * ot_status and leaves a message for ot_last_error() on failure.
+ * Device buffers are caller-owned (hipMalloc'ed or from any allocator) and use
+ * the padded row-major layout described by ot_derived (x fastest).
+ * Threading: a plan is single-stream and not re-entrant; separate plans may run
+ * concurrently on separate streams.
+ */
+#ifndef OT_OT_H
+#define OT_OT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OT_ABI_VERSION 1
+
+typedef enum ot_status {
+  OT_OK = 0,
+  OT_EINVAL = 1,        /* invalid parameter / inconsistent geometry
+                           (reference: assert / "Consistency error!",
+                           opencl/jacobi-3d/jacobi-3d.cpp:105-113) */
+  OT_ERESOURCE = 2,     /* block or LDS too large for the device
+                           (reference: exit 1, opencl/jacobi-2d/jacobi-2d.cpp:682-690) */
+  OT_EHIP = 3,          /* HIP runtime error (reference: CLContext::throwOnError,
+                           opencl/common/src/CLContext.cpp:50-56) */
+  OT_EUNSUPPORTED = 4   /* configuration this build does not implement */
+} ot_status;
+
+/* Stencils. Each one is the overlapped-tiled kernel of one reference driver
+ * (opencl/<name>/<name>.cpp); P3D7 is the 7-point Poisson-Jacobi relaxation with
+ * a read-only right-hand side that has no reference driver. */
+typedef enum ot_stencil {
+  OT_J1D3 = 0,    /* jacobi-1d        0.333f*(l+c+r)                       */
+  OT_J1D5 = 1,    /* jacobi-1d-5pt    0.2f*sum5                            */
+  OT_J1D7 = 2,    /* jacobi-1d-7pt    0.2f*sum7 (sic)                      */
+  OT_J2D = 3,     /* jacobi-2d        0.2f*(l+c+r+t+b)                     */
+  OT_J3D = 4,     /* jacobi-3d        0.143f*sum7                          */
+  OT_P2D = 5,     /* poisson-2d       8c - sum8                            */
+  OT_P3D27 = 6,   /* poisson-3d       26c - sum26                          */
+  OT_P3D7 = 7,    /* Poisson-Jacobi   (sum6 - f)/6, f read-only            */
+  OT_FDTD2D = 8,  /* fdtd-2d          Ex, Ey, Hz                           */
+  OT_G2D = 9,     /* gradient-2d      c + 1/sqrt(1e-4 + sum4 (c-n)^2)      */
+  OT_G3D = 10,    /* gradient-3d      c + 1/sqrt(1e-4 + sum6 (c-n)^2)      */
+  OT_RICIAN2D = 11, /* rician-denoise-2d, f read-only                      */
+  OT_TV2D = 12,   /* cdsc-tv-update-2d, V and F read-only                  */
+  OT_NUM_STENCILS = 13
+} ot_stencil;
+
+enum {
+  OT_GEOM_TILE = 0,       /* block = output tile, halo T*r added, full grid   */
+  OT_GEOM_REFERENCE = 1   /* block includes halo, realSize truncated as in the
+                             reference's computeDerived()                     */
+};
+
+/* variant flags (ot_params.variant) */
+enum {
+  OT_VARIANT_FDTD_PHYSICAL = 1  /* Hz uses -Ey(y) (curl sign); default keeps the
+                                   reference's "+eyCenter" (fdtd-2d.cpp:320-322) */
+};
+
+typedef struct ot_stencil_info {
+  int32_t stencil;
+  int32_t dims;             /* 1, 2 or 3 */
+  int32_t radius;           /* per-time-step dependency radius (Rician: 2) */
+  int32_t num_rw;           /* read-write (ping-pong) fields */
+  int32_t num_ro;           /* read-only fields */
+  int32_t flops_per_point;  /* the driver's "Actual GFlop/s" constant */
+  int32_t loads_per_point;  /* phase2_global_loads key */
+  int32_t ref_padding_mult; /* reference padding = mult * T */
+  const char* name;         /* reference driver name, e.g. "jacobi-2d" */
+} ot_stencil_info;
+
+typedef struct ot_params {
+  int32_t stencil;          /* ot_stencil */
+  int32_t problem_size;     /* -n/--problem-size: edge length of the square/cube/line */
+  int32_t nx, ny, nz;       /* tile geometry only: per-axis extents, 0 = problem_size */
+  int32_t time_steps;       /* -t/--time-steps */
+  int32_t time_tile;        /* -s/--time-tile-size (T) */
+  int32_t block_x;          /* -x/--block-size-x */
+  int32_t block_y;          /* -y/--block-size-y */
+  int32_t block_z;          /* -z/--block-size-z (3D: z-chunk streamed by one workgroup) */
+  int32_t elems_per_thread; /* -e/--elements-per-thread */
+  int32_t geometry;         /* OT_GEOM_TILE | OT_GEOM_REFERENCE */
+  int32_t phase_limit;      /* -p/--phase-limit: 0 full kernel, 1 prologue only,
+                               2 global phase only, 3 on-chip phase only */
+  int32_t variant;          /* OT_VARIANT_* flags */
+  float coef[4];            /* stencil scalars, 0 = reference default:
+                               TV2D: coef[0] = alpha (1.1, cdsc-tv-update-2d.cpp:164)
+                               RICIAN2D: coef[0] = sigma (0.05), coef[1] = lambda
+                               (0.065), coef[2] = dt (5.0), rician-denoise-2d.cpp:154-161 */
+} ot_params;
+
+typedef struct ot_derived {
+  int32_t dims;
+  int32_t radius;
+  int32_t padding;             /* P: ring width of the padded layout */
+  int32_t halo;                /* radius * T: halo loaded around each output tile */
+  int32_t comps_per_block[3];  /* (bx, by*ept, bz) */
+  int32_t real_per_block[3];   /* output points per workgroup per axis */
+  int32_t size_lcm;            /* reference geometry: gcd/lcm of real_per_block */
+  int32_t real_size[3];        /* domain extents (x, y, z); 1 for unused axes */
+  int32_t padded_size[3];      /* real_size + 2P (1D adds the reference's +109) */
+  int32_t num_blocks[3];       /* workgroups per axis as the reference reports them */
+  int32_t tile[3];             /* output points per workgroup this engine uses */
+  int32_t grid_blocks[3];      /* workgroups this engine launches per axis (covers the domain) */
+  int32_t shared_size;         /* reference "Shared Size" key (bytes) */
+  int32_t lds_bytes;           /* LDS this engine's kernel allocates per workgroup */
+  int32_t threads_per_block;   /* this engine's workgroup size */
+  int32_t launches;            /* ceil(time_steps / T) */
+  int32_t num_rw, num_ro;
+  int64_t field_elems;         /* elements of one padded field buffer */
+  int64_t stride_y;            /* elements between rows    (= padded_size[0]) */
+  int64_t stride_z;            /* elements between planes  (= padded_size[0]*padded_size[1]) */
+  int64_t origin;              /* element index of interior point (0,0,0) */
+  double total_fp_per_block;   /* printProgramParameters keys (ProgramGenerator.hpp:46-66) */
+  double useful_fp_per_block;
+  double useful_ratio;
+  double total_fp_steps;        /* "Total FP" / "Useful FP" keys of drivers that scale by
+                                  time_steps instead of T (poisson-2d.cpp, cdsc-tv-update-2d.cpp) */
+  double useful_fp_steps;
+  int64_t global_loads_per_block;
+  int64_t global_stores_per_block;
+  int64_t shared_loads_per_block;
+  int64_t shared_stores_per_block;
+} ot_derived;
+
+typedef struct ot_timing {
+  double wall_s;          /* host wall clock around the launch loop ("Elapsed Time") */
+  double event_s;         /* HIP events first launch start -> last launch end ("EventElapsed") */
+  int32_t launches;       /* kernel launches issued */
+  int32_t final_buffer;   /* 0: result is in the rw_a buffers, 1: in rw_b */
+} ot_timing;
+
+typedef struct ot_plan ot_plan;
+
+/* ---- library / device ---------------------------------------------------- */
+const char* ot_version(void);
+const char* ot_last_error(void);
+/* Device properties of the current HIP device (no-op error on a host without GPU). */
+ot_status ot_device_query(int32_t* num_cus, int32_t* lds_per_block, int32_t* max_threads,
+                          char* name, int32_t name_len);
+
+/* ---- stencil catalogue --------------------------------------------------- */
+ot_status ot_stencil_query(int32_t stencil, ot_stencil_info* info);
+ot_status ot_stencil_from_name(const char* name, int32_t* stencil);
+
+/* ---- geometry (pure host code; replaces GeneratorParams::computeDerived,
+ *      e.g. opencl/jacobi-2d/jacobi-2d.cpp:84-126, jacobi-3d.cpp:81-115) -------- */
+ot_status ot_params_default(int32_t stencil, ot_params* params);
+ot_status ot_derive(const ot_params* params, ot_derived* derived);
+
+/* ---- plans (replace XGenerator::generate + cl::Program::build,
+ *      opencl/jacobi-2d/jacobi-2d.cpp:162-173, :741-760) ----------------------- */
+ot_status ot_plan_create(const ot_params* params, ot_plan** plan);
+ot_status ot_plan_derived(const ot_plan* plan, ot_derived* derived);
+void ot_plan_destroy(ot_plan* plan);
+
+/* Run the whole time loop (replaces opencl/jacobi-2d/jacobi-2d.cpp:903-933):
+ * ceil(steps/T) launches ping-ponging between the rw_a and rw_b field sets,
+ * the last launch guarded to the remaining steps. rw_a holds the initial state;
+ * read-only fields (ro) are never swapped. stream is a hipStream_t (NULL = the
+ * null stream). use_graph != 0 captures the loop once into a hipGraph and
+ * replays it on later calls with the same buffers. On return the stream is
+ * synchronised and timing (optional) is filled. */
+ot_status ot_run(ot_plan* plan, float* const* rw_a, float* const* rw_b,
+                 const float* const* ro, void* stream, int32_t use_graph,
+                 ot_timing* timing);
+
+/* Enqueue exactly one overlapped-tile launch advancing min(T, steps-base_time)
+ * steps from src to dst (the generated kernel_func(input, aux..., output,
+ * baseTime) of the reference). Asynchronous on stream. */
+ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_dst,
+                    const float* const* ro, int32_t base_time, void* stream);
+
+/* ---- device memory helpers (thin wrappers over the HIP runtime) --------- */
+ot_status ot_dev_alloc(size_t bytes, void** ptr);
+ot_status ot_dev_free(void* ptr);
+ot_status ot_memcpy_h2d(void* dst, const void* src, size_t bytes);
+ot_status ot_memcpy_d2h(void* dst, const void* src, size_t bytes);
+ot_status ot_memcpy_d2d(void* dst, const void* src, size_t bytes);
+ot_status ot_dev_memset(void* ptr, int32_t value, size_t bytes);
+ot_status ot_dev_sync(void);
+
+/* ---- seeded input generators (host; padded layout of ot_derived) ------------
+ * ot_fill_reference_uniform: the reference drivers' init — srand(seed) then
+ * (float)rand()/((float)RAND_MAX+1.0f) for every interior point in row-major
+ * padded order, 0 in the padding ring (opencl/poisson-2d/poisson-2d.cpp:611-621).
+ * Counter-based generators (seed, index) -> value are used for the larger
+ * synthetic workloads, so any sub-block can be generated independently. */
+ot_status ot_fill_reference_uniform(float* host, const ot_derived* d, uint32_t seed);
+ot_status ot_fill_uniform(float* host, const ot_derived* d, uint64_t seed, float lo, float hi);
+ot_status ot_fill_sinusoid_noise(float* host, const ot_derived* d, uint64_t seed, float noise_amp);
+ot_status ot_fill_gaussian_pulse(float* host, const ot_derived* d, float amplitude, float sigma);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OT_OT_H */
