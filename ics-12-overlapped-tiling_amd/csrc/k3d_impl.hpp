@@ -99,6 +99,20 @@ __global__ void __launch_bounds__(1024) k3d(LaunchGeom g, FieldPtrs f) {
   const bool do_store = g.phase_limit == 0;
   const int s_end = (g.phase_limit == 2) ? 1 : T;
 
+  // Level-0 planes are prefetched one iteration ahead into registers, so the
+  // HBM latency of plane k+1 overlaps the stage computations of plane k.
+  float pf[E], pfr[E];
+  auto fetch = [&](int k) {
+    const bool inz = k >= 0 && k < g.n[2];
+    const int64_t zoff = (int64_t)k * g.sz;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      pf[i] = (do_loads && inz && inxy[i]) ? f.src[0][col[i] + zoff] : 0.0f;
+      if constexpr (RHS) pfr[i] = (inz && inxy[i]) ? f.ro[0][col[i] + zoff] : 0.0f;
+    }
+  };
+  fetch(kstart);
+
   for (int k = kstart; k <= kend; ++k) {
 #pragma unroll
     for (int l = 0; l < T; ++l)
@@ -115,18 +129,16 @@ __global__ void __launch_bounds__(1024) k3d(LaunchGeom g, FieldPtrs f) {
     }
     __syncthreads();
 
-    // ---- level 0: plane k from HBM
+    // ---- level 0: plane k (prefetched), then start fetching plane k+1
     if (k <= kload_end) {
-      const bool inz = k >= 0 && k < g.n[2];
-      const int64_t zoff = (int64_t)k * g.sz;
       float* D = slot(0, k);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        const float v = (do_loads && inz && inxy[i]) ? f.src[0][col[i] + zoff] : 0.0f;
-        q[0][2][i] = v;
-        D[(ty * E + i) * FX + tx] = v;
-        if constexpr (RHS) fq[0][i] = (inz && inxy[i]) ? f.ro[0][col[i] + zoff] : 0.0f;
+        q[0][2][i] = pf[i];
+        D[(ty * E + i) * FX + tx] = pf[i];
+        if constexpr (RHS) fq[0][i] = pfr[i];
       }
+      if (k + 1 <= kload_end) fetch(k + 1);
     }
 
     // ---- stages 1..T

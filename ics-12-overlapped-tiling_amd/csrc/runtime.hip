@@ -104,6 +104,9 @@ struct ot_plan {
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
   const float* graph_key[8] = {};
+  // per-launch events (OT_RUN_TIME_LAUNCHES)
+  hipEvent_t* lev = nullptr;
+  int nlev = 0;
 };
 
 static void fill_ptrs(const ot_plan* pl, int launch, float* const* rw_a, float* const* rw_b,
@@ -117,16 +120,20 @@ static void fill_ptrs(const ot_plan* pl, int launch, float* const* rw_a, float* 
   for (int k = 0; k < pl->d.num_ro; ++k) f.ro[k] = ro[k];
 }
 
+// Enqueue the whole time loop. With lev (2 events per launch) every launch is
+// bracketed by its own event pair.
 static ot_status enqueue_loop(ot_plan* pl, float* const* rw_a, float* const* rw_b, const float* const* ro,
-                              hipStream_t s) {
+                              hipStream_t s, hipEvent_t* lev) {
   const int T = pl->p.time_tile;
   for (int l = 0; l < pl->d.launches; ++l) {
     const int base = l * T;
     const int t_eff = (pl->p.time_steps - base) < T ? (pl->p.time_steps - base) : T;
     FieldPtrs f;
     fill_ptrs(pl, l, rw_a, rw_b, ro, f);
+    if (lev) OT_HIP(hipEventRecord(lev[2 * l], s));
     ot_status st = launch_any(pl->p.stencil, pl->d.dims, t_eff, pl->p.elems_per_thread, pl->g, f, s, false);
     if (st != OT_OK) return st;
+    if (lev) OT_HIP(hipEventRecord(lev[2 * l + 1], s));
   }
   return OT_OK;
 }
@@ -215,6 +222,8 @@ void ot_plan_destroy(ot_plan* plan) {
   if (plan->ev0) (void)hipEventDestroy(plan->ev0);
   if (plan->ev1) (void)hipEventDestroy(plan->ev1);
   if (plan->own_stream) (void)hipStreamDestroy(plan->own_stream);
+  for (int i = 0; i < plan->nlev; ++i) (void)hipEventDestroy(plan->lev[i]);
+  delete[] plan->lev;
   delete plan;
 }
 
@@ -241,10 +250,25 @@ ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_
 }
 
 ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const float* const* ro,
-                 void* stream, int32_t use_graph, ot_timing* timing) {
+                 void* stream, int32_t mode, ot_timing* timing) {
   if (!pl) return set_error(OT_EINVAL, "null plan");
   ot_status st = check_buffers(pl, rw_a, rw_b, ro);
   if (st != OT_OK) return st;
+  const bool use_graph = (mode & OT_RUN_GRAPH) != 0;
+  const bool time_launches = (mode & OT_RUN_TIME_LAUNCHES) != 0;
+  if (use_graph && time_launches)
+    return set_error(OT_EINVAL, "OT_RUN_TIME_LAUNCHES times eager launches; do not combine with OT_RUN_GRAPH");
+  if (time_launches && pl->nlev < 2 * pl->d.launches) {
+    for (int i = 0; i < pl->nlev; ++i) (void)hipEventDestroy(pl->lev[i]);
+    delete[] pl->lev;
+    pl->nlev = 0;
+    pl->lev = new (std::nothrow) hipEvent_t[2 * pl->d.launches];
+    if (!pl->lev) return set_error(OT_ERESOURCE, "out of host memory");
+    for (int i = 0; i < 2 * pl->d.launches; ++i) {
+      OT_HIP(hipEventCreate(&pl->lev[i]));
+      pl->nlev = i + 1;
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   if (!s && use_graph) {
     if (!pl->own_stream) OT_HIP(hipStreamCreateWithFlags(&pl->own_stream, hipStreamNonBlocking));
@@ -275,7 +299,7 @@ ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const floa
         pl->exec = nullptr;
       }
       OT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      ot_status cst = enqueue_loop(pl, rw_a, rw_b, ro, s);
+      ot_status cst = enqueue_loop(pl, rw_a, rw_b, ro, s, nullptr);
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(s, &graph);
       if (cst != OT_OK) {
@@ -295,7 +319,7 @@ ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const floa
   if (use_graph) {
     OT_HIP(hipGraphLaunch(pl->exec, s));
   } else {
-    st = enqueue_loop(pl, rw_a, rw_b, ro, s);
+    st = enqueue_loop(pl, rw_a, rw_b, ro, s, time_launches ? pl->lev : nullptr);
     if (st != OT_OK) return st;
   }
   OT_HIP(hipEventRecord(pl->ev1, s));
@@ -305,6 +329,15 @@ ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const floa
   OT_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
   tm.wall_s = std::chrono::duration<double>(t1 - t0).count();
   tm.event_s = ms * 1e-3;
+  tm.full_launches = pl->p.time_steps / pl->p.time_tile;
+  if (time_launches) {
+    for (int l = 0; l < pl->d.launches; ++l) {
+      float lms = 0.0f;
+      OT_HIP(hipEventElapsedTime(&lms, pl->lev[2 * l], pl->lev[2 * l + 1]));
+      tm.kernel_s += lms * 1e-3;
+      if (l < tm.full_launches) tm.kernel_full_s += lms * 1e-3;
+    }
+  }
   if (timing) *timing = tm;
   return OT_OK;
 }

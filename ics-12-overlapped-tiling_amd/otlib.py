@@ -64,8 +64,13 @@ class Derived(C.Structure):
 
 
 class Timing(C.Structure):
-    _fields_ = [("wall_s", C.c_double), ("event_s", C.c_double), ("launches", C.c_int32),
-                ("final_buffer", C.c_int32)]
+    _fields_ = [("wall_s", C.c_double), ("event_s", C.c_double), ("kernel_s", C.c_double),
+                ("kernel_full_s", C.c_double), ("launches", C.c_int32),
+                ("full_launches", C.c_int32), ("final_buffer", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+RUN_GRAPH, RUN_TIME_LAUNCHES = 1, 2
 
 
 class StencilInfo(C.Structure):
@@ -276,13 +281,14 @@ class Plan:
         self.derived = Derived()
         check(lib().ot_plan_derived(h, C.byref(self.derived)))
 
-    def run(self, rw_a, rw_b, ro=(), use_graph=False, stream=None) -> Timing:
+    def run(self, rw_a, rw_b, ro=(), use_graph=False, time_launches=False, stream=None) -> Timing:
         """rw_a/rw_b/ro: sequences of DeviceBuffer. Returns the Timing."""
         a = _ptr_array([b.ptr for b in rw_a])
         b = _ptr_array([x.ptr for x in rw_b])
         r = _ptr_array([x.ptr for x in ro]) if ro else None
         tm = Timing()
-        check(lib().ot_run(self.handle, a, b, r, stream, 1 if use_graph else 0, C.byref(tm)))
+        mode = (RUN_GRAPH if use_graph else 0) | (RUN_TIME_LAUNCHES if time_launches else 0)
+        check(lib().ot_run(self.handle, a, b, r, stream, mode, C.byref(tm)))
         return tm
 
     def launch(self, src, dst, ro=(), base_time=0, stream=None) -> None:

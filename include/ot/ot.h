@@ -144,9 +144,19 @@ typedef struct ot_derived {
 typedef struct ot_timing {
   double wall_s;          /* host wall clock around the launch loop ("Elapsed Time") */
   double event_s;         /* HIP events first launch start -> last launch end ("EventElapsed") */
+  double kernel_s;        /* OT_RUN_TIME_LAUNCHES: sum of per-launch event durations */
+  double kernel_full_s;   /* ... of the launches advancing a full T steps */
   int32_t launches;       /* kernel launches issued */
+  int32_t full_launches;  /* launches advancing a full T steps */
   int32_t final_buffer;   /* 0: result is in the rw_a buffers, 1: in rw_b */
+  int32_t reserved;
 } ot_timing;
+
+/* ot_run mode bits */
+enum {
+  OT_RUN_GRAPH = 1,        /* capture the loop into a hipGraph once, replay it */
+  OT_RUN_TIME_LAUNCHES = 2 /* bracket every launch with HIP events (eager only) */
+};
 
 typedef struct ot_plan ot_plan;
 
@@ -176,11 +186,13 @@ void ot_plan_destroy(ot_plan* plan);
  * ceil(steps/T) launches ping-ponging between the rw_a and rw_b field sets,
  * the last launch guarded to the remaining steps. rw_a holds the initial state;
  * read-only fields (ro) are never swapped. stream is a hipStream_t (NULL = the
- * null stream). use_graph != 0 captures the loop once into a hipGraph and
- * replays it on later calls with the same buffers. On return the stream is
- * synchronised and timing (optional) is filled. */
+ * null stream). mode is a mask of OT_RUN_* bits: OT_RUN_GRAPH captures the
+ * loop once into a hipGraph and replays it on later calls with the same
+ * buffers and stream; OT_RUN_TIME_LAUNCHES times every launch with its own
+ * event pair. On return the stream is synchronised and timing (optional) is
+ * filled. */
 ot_status ot_run(ot_plan* plan, float* const* rw_a, float* const* rw_b,
-                 const float* const* ro, void* stream, int32_t use_graph,
+                 const float* const* ro, void* stream, int32_t mode,
                  ot_timing* timing);
 
 /* Enqueue exactly one overlapped-tile launch advancing min(T, steps-base_time)
