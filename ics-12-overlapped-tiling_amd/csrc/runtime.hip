@@ -48,7 +48,8 @@ void shape_3d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* 
   const int FX = tile[0] + 2 * T, FY = tile[1] + 2 * T;
   *bx = FX;
   *by = (FY + ept - 1) / ept;
-  *lds = T * NS * (*by) * ept * FX * (int)sizeof(float);
+  // T levels x NS ring planes + one guard row (+1) before and after
+  *lds = (T * NS * (*by) * ept * FX + 2 * (FX + 1)) * (int)sizeof(float);
 }
 
 ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
