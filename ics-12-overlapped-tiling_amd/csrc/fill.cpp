@@ -8,6 +8,7 @@
 // library's rand(), i.e. glibc's generator, as the reference does.
 // The other generators are counter-based (splitmix64 of seed and the interior
 // linear index), so any sub-block of a large grid can be produced independently.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -91,6 +92,44 @@ ot_status ot_fill_sinusoid_noise(float* host, const ot_derived* d, uint64_t seed
     const double noise = (double)amp * (2.0 * (double)u01(seed, (uint64_t)((z * ny + y) * nx + x)) - 1.0);
     host[i] = (float)(v + noise);
   });
+  return OT_OK;
+}
+
+ot_status ot_fill_rician_phantom(float* arena, int32_t n_images, const ot_image* images, uint64_t seed,
+                                 float sigma) {
+  if (!arena || !images || n_images < 1) return ot::set_error(OT_EINVAL, "bad arguments");
+  for (int32_t m = 0; m < n_images; ++m) {
+    const ot_image& im = images[m];
+    const uint64_t s = splitmix64(seed ^ (0x51ed2701ull * (uint64_t)(m + 1)));
+    uint64_t ctr = 0;
+    auto uni = [&]() { return u01(s, ctr++); };
+    const int w = im.width, h = im.height;
+    float* img = arena + im.offset;
+    // background plus 8..32 rectangles, each with one intensity in [0, 1)
+    const float bg = uni();
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < im.pitch; ++x) img[(int64_t)y * im.pitch + x] = (x < w) ? bg : 0.0f;
+    const int nrect = 8 + (int)(uni() * 25.0f);
+    for (int r = 0; r < nrect; ++r) {
+      int xa = (int)(uni() * w), xb = (int)(uni() * w), ya = (int)(uni() * h), yb = (int)(uni() * h);
+      if (xa > xb) std::swap(xa, xb);
+      if (ya > yb) std::swap(ya, yb);
+      const float v = uni();
+      for (int y = ya; y <= yb && y < h; ++y)
+        for (int x = xa; x <= xb && x < w; ++x) img[(int64_t)y * im.pitch + x] = v;
+    }
+    // Rician corruption: |p + n1 + i n2|, n1, n2 ~ N(0, sigma) (Box-Muller)
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) {
+        const double u1 = std::max(1e-12, (double)uni()), u2 = (double)uni();
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        const double n1 = sigma * rad * std::cos(6.283185307179586 * u2);
+        const double n2 = sigma * rad * std::sin(6.283185307179586 * u2);
+        float& p = img[(int64_t)y * im.pitch + x];
+        const double re = (double)p + n1;
+        p = (float)std::sqrt(re * re + n2 * n2);
+      }
+  }
   return OT_OK;
 }
 

@@ -37,9 +37,14 @@ ot_status set_error(ot_status st, const char* fmt, ...) {
     if (e_ != hipSuccess) return set_error(OT_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
+// VGPR count of the kernel most recently prepared on this thread.
+static thread_local int g_prepared_regs = 0;
+
 ot_status prepare_lds(const void* kernel, int lds_bytes) {
   if (lds_bytes > 65536)
     OT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
+  hipFuncAttributes attr;
+  if (hipFuncGetAttributes(&attr, kernel) == hipSuccess) g_prepared_regs = attr.numRegs;
   return OT_OK;
 }
 
@@ -200,7 +205,9 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   std::memset(&f, 0, sizeof(f));
   const int T = params->time_tile;
   const int last = params->time_steps % T;
+  g_prepared_regs = 0;
   st = launch_any(params->stencil, d.dims, T, params->elems_per_thread, g, f, nullptr, true);
+  pl->d.registers = g_prepared_regs;
   if (st == OT_OK && last > 0)
     st = launch_any(params->stencil, d.dims, last, params->elems_per_thread, g, f, nullptr, true);
   if (st != OT_OK) {

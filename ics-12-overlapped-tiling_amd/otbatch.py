@@ -1,0 +1,87 @@
+"""Batches of independent 2D images through the engine's C ABI (ot_batch_*).
+
+A batch packs images of uneven size into one arena per field; every
+time-tile launch is one persistent kernel draining a device work queue of
+(image, tile) items, and the whole loop can be replayed from a hipGraph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import otlib
+from otlib import DeviceBuffer, Image, Params, Timing, _ptr_array, check, lib
+
+
+def layout(widths, heights):
+    """(images ctypes array, arena elements) for the given extents."""
+    n = len(widths)
+    w = (C.c_int32 * n)(*map(int, widths))
+    h = (C.c_int32 * n)(*map(int, heights))
+    ims = (Image * n)()
+    total = C.c_int64()
+    check(lib().ot_batch_layout(n, w, h, ims, C.byref(total)))
+    return ims, int(total.value)
+
+
+def phantom_arena(images, total: int, seed: int, sigma: float = 0.05) -> np.ndarray:
+    a = np.zeros(total, np.float32)
+    check(lib().ot_fill_rician_phantom(a.ctypes.data_as(C.POINTER(C.c_float)), len(images), images, seed, sigma))
+    return a
+
+
+def image_view(arena: np.ndarray, im: Image) -> np.ndarray:
+    """(height, width) view of one image inside an arena."""
+    return arena[im.offset: im.offset + im.pitch * im.height].reshape(im.height, im.pitch)[:, :im.width]
+
+
+def draw_sizes(n: int, seed: int, wmin=64, wmax=1024, hmin=64, hmax=768):
+    rng = np.random.default_rng(seed)
+    return rng.integers(wmin, wmax + 1, n), rng.integers(hmin, hmax + 1, n)
+
+
+class Batch:
+    def __init__(self, params: Params, images):
+        self.params = params
+        self.images = images
+        h = C.c_void_p()
+        check(lib().ot_batch_create(C.byref(params), len(images), images, C.byref(h)))
+        self.handle = h
+        items, wgs, launches = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().ot_batch_info(h, C.byref(items), C.byref(wgs), C.byref(launches)))
+        self.items, self.workgroups, self.launches = items.value, wgs.value, launches.value
+
+    def run(self, rw_a, rw_b, ro=(), use_graph=False, stream=None) -> Timing:
+        a = _ptr_array([b.ptr for b in rw_a])
+        b = _ptr_array([x.ptr for x in rw_b])
+        r = _ptr_array([x.ptr for x in ro]) if ro else None
+        tm = Timing()
+        check(lib().ot_batch_run(self.handle, a, b, r, stream, otlib.RUN_GRAPH if use_graph else 0, C.byref(tm)))
+        return tm
+
+    def close(self):
+        if self.handle:
+            lib().ot_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_batch_host(params: Params, images, rw_arenas, ro_arenas=(), use_graph=False):
+    """Upload arenas, run the batch, download the rw arenas."""
+    bt = Batch(params, images)
+    a = [DeviceBuffer.from_array(x) for x in rw_arenas]
+    b = [DeviceBuffer.from_array(x) for x in rw_arenas]
+    ro = [DeviceBuffer.from_array(x) for x in ro_arenas]
+    tm = bt.run(a, b, ro, use_graph=use_graph)
+    src = a if tm.final_buffer == 0 else b
+    out = [buf.download(x.shape) for buf, x in zip(src, rw_arenas)]
+    for buf in a + b + ro:
+        buf.free()
+    bt.close()
+    return out, tm

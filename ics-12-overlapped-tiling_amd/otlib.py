@@ -60,6 +60,7 @@ class Derived(C.Structure):
         ("useful_fp_steps", C.c_double),
         ("global_loads_per_block", C.c_int64), ("global_stores_per_block", C.c_int64),
         ("shared_loads_per_block", C.c_int64), ("shared_stores_per_block", C.c_int64),
+        ("registers", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 
@@ -71,6 +72,11 @@ class Timing(C.Structure):
 
 
 RUN_GRAPH, RUN_TIME_LAUNCHES = 1, 2
+
+
+class Image(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pitch", C.c_int32),
+                ("reserved", C.c_int32), ("offset", C.c_int64)]
 
 
 class StencilInfo(C.Structure):
@@ -97,6 +103,14 @@ SIGNATURES = [
     ("ot_plan_destroy", None, [C.c_void_p]),
     ("ot_run", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_void_p, C.c_int32, C.POINTER(Timing)]),
     ("ot_launch", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_int32, C.c_void_p]),
+    ("ot_naive_run", C.c_int, [C.POINTER(Params), _PP, _PP, _PP, C.c_void_p, C.POINTER(Timing)]),
+    ("ot_batch_layout", C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(Image),
+                                  C.POINTER(C.c_int64)]),
+    ("ot_batch_create", C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(Image), C.POINTER(C.c_void_p)]),
+    ("ot_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("ot_batch_run", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_void_p, C.c_int32, C.POINTER(Timing)]),
+    ("ot_batch_destroy", None, [C.c_void_p]),
+    ("ot_fill_rician_phantom", C.c_int, [_FP, C.c_int32, C.POINTER(Image), C.c_uint64, C.c_float]),
     ("ot_dev_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     ("ot_dev_free", C.c_int, [C.c_void_p]),
     ("ot_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -336,4 +350,23 @@ def run_host(params: Params, rw_init, ro_init=(), use_graph=False, repeat=1) -> 
     for buf in a + b + ro:
         buf.free()
     plan.close()
+    return RunResult(out, tm, d)
+
+
+def run_naive_host(params: Params, rw_init, ro_init=()) -> RunResult:
+    """The same problem through the non-tiled one-step-per-launch kernels."""
+    d = derive(params)
+    shape = layout_shape(d)
+    a = [DeviceBuffer.from_array(np.ascontiguousarray(x, dtype=np.float32)) for x in rw_init]
+    b = [DeviceBuffer.from_array(np.ascontiguousarray(x, dtype=np.float32)) for x in rw_init]
+    ro = [DeviceBuffer.from_array(np.ascontiguousarray(x, dtype=np.float32)) for x in ro_init]
+    pa = _ptr_array([x.ptr for x in a])
+    pb = _ptr_array([x.ptr for x in b])
+    pr = _ptr_array([x.ptr for x in ro]) if ro else None
+    tm = Timing()
+    check(lib().ot_naive_run(C.byref(params), pa, pb, pr, None, C.byref(tm)))
+    src = a if tm.final_buffer == 0 else b
+    out = [buf.download(shape) for buf in src]
+    for buf in a + b + ro:
+        buf.free()
     return RunResult(out, tm, d)

@@ -139,6 +139,9 @@ typedef struct ot_derived {
   int64_t global_stores_per_block;
   int64_t shared_loads_per_block;
   int64_t shared_stores_per_block;
+  int32_t registers;           /* VGPRs per lane of the plan's main kernel ("Register
+                                  Usage"); 0 from ot_derive, filled by ot_plan_create */
+  int32_t reserved0;
 } ot_derived;
 
 typedef struct ot_timing {
@@ -200,6 +203,51 @@ ot_status ot_run(ot_plan* plan, float* const* rw_a, float* const* rw_b,
  * baseTime) of the reference). Asynchronous on stream. */
 ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_dst,
                     const float* const* ro, int32_t base_time, void* stream);
+
+/* Run the same problem with straightforward one-step-per-launch kernels (no
+ * tiling, every tap from global memory): the counterpart of the reference CUDA
+ * prototypes' *_notimetiling kernels (cuda/jacobi-2d/jacobi-2d.cu:88-105),
+ * used by the drivers' --verify and as the no-tiling baseline. params gives
+ * the grid and steps (time_tile and blocks only shape the layout). rw_a holds
+ * the initial state; timing->final_buffer says where the result is. */
+ot_status ot_naive_run(const ot_params* params, float* const* rw_a, float* const* rw_b,
+                       const float* const* ro, void* stream, ot_timing* timing);
+
+/* ---- batches of independent 2D images --------------------------------------
+ * No reference counterpart (the reference runs one square grid per process).
+ * Many images of uneven size (e.g. Rician or TV denoising of a batch of
+ * differently sized images) advance together: each time-tile launch is one
+ * persistent kernel whose workgroups pull (image, tile) items from a device
+ * work queue, and the whole time loop can be replayed from one hipGraph.
+ * Images live in arenas (one per field) with a per-image offset and pitch. */
+typedef struct ot_image {
+  int32_t width, height;  /* domain extents (x, y) */
+  int32_t pitch;          /* elements between rows (>= width) */
+  int32_t reserved;
+  int64_t offset;         /* element index of pixel (0,0) inside the arena */
+} ot_image;
+
+typedef struct ot_batch ot_batch;
+
+/* Pack images into one arena: pitch = width rounded up to 16 elements, every
+ * image starting 64-byte aligned. images (out) has n_images entries. */
+ot_status ot_batch_layout(int32_t n_images, const int32_t* widths, const int32_t* heights,
+                          ot_image* images, int64_t* arena_elems);
+/* params: a 2D stencil with time_steps, time_tile, block_x, block_y,
+ * elems_per_thread, coef, variant (problem sizes are ignored). */
+ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_image* images,
+                          ot_batch** batch);
+ot_status ot_batch_info(const ot_batch* batch, int32_t* items, int32_t* workgroups,
+                        int32_t* launches);
+/* rw_a/rw_b/ro: arena pointers per field, as in ot_run; mode: OT_RUN_* bits. */
+ot_status ot_batch_run(ot_batch* batch, float* const* rw_a, float* const* rw_b,
+                       const float* const* ro, void* stream, int32_t mode, ot_timing* timing);
+void ot_batch_destroy(ot_batch* batch);
+/* Seeded test images for the batch workload: per image a piecewise-constant
+ * phantom of 8..32 random rectangles with intensities in [0,1], corrupted by
+ * Rician noise |phantom + N(0,sigma) + i N(0,sigma)|. Writes the arena. */
+ot_status ot_fill_rician_phantom(float* arena, int32_t n_images, const ot_image* images,
+                                 uint64_t seed, float sigma);
 
 /* ---- device memory helpers (thin wrappers over the HIP runtime) --------- */
 ot_status ot_dev_alloc(size_t bytes, void** ptr);

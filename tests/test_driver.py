@@ -1,0 +1,96 @@
+"""The ot-<stencil> drivers: reference CLI and stdout wire format.
+
+CPU tests cover option handling; GPU tests run the drivers with --verify and
+check their derived keys against the reference's recorded sweep rows."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "ics-12-overlapped-tiling_amd", "bin")
+
+
+def run_driver(name, *args, timeout=120):
+    r = subprocess.run([os.path.join(BIN, f"ot-{name}"), *map(str, args)], capture_output=True, text=True,
+                       timeout=timeout)
+    keys = {}
+    for line in r.stdout.splitlines():
+        if ": " in line:
+            k, v = line.split(": ", 1)
+            keys[k] = v
+    return r.returncode, keys, r
+
+
+def test_driver_help_exits_1():
+    rc, _, r = run_driver("jacobi-2d", "--help")
+    assert rc == 1 and "--time-tile-size" in r.stderr
+
+
+def test_driver_rejects_unknown_option():
+    rc, _, r = run_driver("jacobi-3d", "--bogus", "1")
+    assert rc == 1
+
+
+def test_every_stencil_has_a_driver(otlib):
+    for name in otlib.STENCILS:
+        assert os.path.exists(os.path.join(BIN, f"ot-{name}")), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,args", [
+    ("jacobi-2d", ["-n", 1024, "-t", 64, "-s", 4, "-x", 16, "-y", 16]),
+    ("jacobi-3d", ["-n", 64, "-t", 9, "-s", 2, "-x", 8, "-y", 8, "-z", 8]),
+    ("jacobi-1d", ["-n", 5000, "-t", 17, "-s", 3, "-x", 64, "-e", 2]),
+    ("jacobi-1d-5pt", ["-n", 3000, "-t", 9, "-s", 2, "-x", 64]),
+    ("jacobi-1d-7pt", ["-n", 3000, "-t", 9, "-s", 2, "-x", 64]),
+    ("poisson-2d", ["-n", 200, "-t", 8, "-s", 2, "-x", 16, "-y", 8]),
+    ("poisson-3d", ["-n", 40, "-t", 4, "-s", 2, "-x", 8, "-y", 8, "-z", 8]),
+    ("poisson-3d-rhs", ["-n", 40, "-t", 9, "-s", 3, "-x", 16, "-y", 16, "-z", 8, "--geometry", "tile"]),
+    ("gradient-2d", ["-n", 300, "-t", 16, "-s", 4, "-x", 32, "-y", 8, "-e", 2]),
+    ("gradient-3d", ["-n", 50, "-t", 8, "-s", 2, "-x", 8, "-y", 8, "-z", 8]),
+    ("fdtd-2d", ["-n", 300, "-t", 20, "-s", 5, "-x", 32, "-y", 16]),
+    ("rician-denoise-2d", ["-n", 300, "-t", 20, "-s", 2, "-x", 32, "-y", 8]),
+    ("cdsc-tv-update-2d", ["-n", 300, "-t", 20, "-s", 4, "-x", 32, "-y", 16, "--graph"]),
+])
+def test_driver_verify_passes(name, args):
+    rc, keys, r = run_driver(name, *args, "-v")
+    print(r.stdout[-600:], flush=True)
+    assert rc == 0, r.stdout + r.stderr
+    assert keys["Correctness"] == "PASSED"
+    assert float(keys["Max Relative Error"]) <= 1e-5
+    for k in ("EventElapsed", "Elapsed Time", "Actual GFlop/s", "Device GFlop/s", "Register Usage",
+              "Num Blocks X", "Padded Size", "Real Size", "phase2_global_loads", "Dimensions"):
+        assert k in keys, k
+
+
+@pytest.mark.gpu
+def test_driver_keys_match_reference_sweep_rows():
+    """A few rows of the reference's jacobi-2d C2050 sweep: the driver prints the
+    same derived keys for the same command line."""
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "geometry_c2050.json")))
+    rows = [r for r in golden if r["stencil"] == "jacobi-2d"][:4] + \
+           [r for r in golden if r["stencil"] == "jacobi-3d"][:3]
+    for r in rows:
+        args = ["-n", r["problem_size"], "-t", 2, "-s", r["Time Tile Size"], "-x", r["Block Size X"],
+                "-y", r["Block Size Y"], "-e", r["Elements/Thread"]]
+        if r["stencil"] == "jacobi-3d":
+            args += ["-z", r["Block Size Z"]]
+        rc, keys, out = run_driver(r["stencil"], *args, timeout=300)
+        assert rc == 0, out.stdout + out.stderr
+        for k in ("Num Blocks X", "Num Blocks Y", "Num Blocks Z", "Padded Size", "Shared Size",
+                  "Global Loads/Block", "Shared Loads/Block"):
+            if k in r:
+                assert int(keys[k]) == int(r[k]), (k, r)
+
+
+@pytest.mark.gpu
+def test_save_and_load_plan(tmp_path):
+    plan = tmp_path / "plan.txt"
+    rc, k1, _ = run_driver("jacobi-2d", "-n", 500, "-t", 8, "-s", 2, "-w", plan)
+    assert rc == 0 and plan.exists()
+    rc, k2, _ = run_driver("jacobi-2d", "-f", plan, "-v")
+    assert rc == 0 and k2["Correctness"] == "PASSED"
+    assert k1["Num Blocks X"] == k2["Num Blocks X"] and k1["Time Tile Size"] == k2["Time Tile Size"]
