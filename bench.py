@@ -34,6 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # sweep (T, block x, block y rows, block z, elements per thread) and T=1.
 DEFAULT_TILED = dict(T=2, x=32, y=4, z=32, e=4)
 DEFAULT_T1 = dict(T=1, x=64, y=4, z=32, e=4)
+# batch workloads: (T, block x, block y rows, elements per thread)
+DEFAULT_BATCH = {"batch-rician": (3, 32, 8, 4), "batch-tv": (2, 64, 4, 4)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jacobi3d_256_traffic.json")
 
 
@@ -51,6 +53,12 @@ def parse():
     ap.add_argument("--ept", type=int, default=DEFAULT_TILED["e"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-t1", action="store_true", help="skip the T=1 comparison")
+    ap.add_argument("--workload", default="jacobi3d",
+                    choices=["jacobi3d", "batch-rician", "batch-tv"],
+                    help="jacobi3d: the headline line; batch-*: 256 uneven images, batched persistent "
+                         "launches vs one launch per image and time tile")
+    ap.add_argument("--images", type=int, default=256)
+    ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes) and exit")
@@ -117,10 +125,70 @@ def cpu_baseline(n, steps_t):
     }
 
 
+def bench_batch(args, stencil_name):
+    """Batched Rician / TV denoising of uneven images: one persistent
+    work-queue launch per time tile (hipGraph replay) against the same images
+    advanced by one per-grid launch per image and time tile."""
+    import otbatch
+
+    st = otlib.RICIAN2D if stencil_name == "rician" else otlib.TV2D
+    widths, heights = otbatch.draw_sizes(args.images, seed=77)
+    ims, total = otbatch.layout(widths, heights)
+    f = otbatch.phantom_arena(ims, total, seed=77, sigma=0.05)
+    coef = [0.05, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
+    T, bx, by, e = (args.T, args.bx, args.by, args.ept)
+    p = otlib.make_params(st, steps=args.iterations, T=T, x=bx, y=by, e=e, coef=coef)
+    ro_host = [f] if st == otlib.RICIAN2D else [np.ones(total, np.float32), f]
+    A = otlib.DeviceBuffer.from_array(f)
+    B = otlib.DeviceBuffer.from_array(f)
+    RO = [otlib.DeviceBuffer.from_array(x) for x in ro_host]
+    bt = otbatch.Batch(p, ims)
+
+    def timed(use_graph, per_image, reps):
+        for _ in range(args.warmup):
+            bt.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
+        otlib.check(otlib.lib().ot_dev_sync())
+        return (time.perf_counter() - t0) / reps
+
+    batched = timed(True, False, args.steps)
+    batched_eager = timed(False, False, args.steps)
+    per_image = timed(False, True, max(2, args.steps // 2))
+    per_image_graph = timed(True, True, max(2, args.steps // 2))
+    pixels = float(sum(int(im.width) * int(im.height) for im in ims))
+    out = {
+        "metric": f"batched {stencil_name} denoise Gpixel-iterations/s",
+        "value": round(pixels * args.iterations / batched / 1e9, 3),
+        "unit": "Gpixel-iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(batched * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: rectangle phantoms + Rician noise sigma 0.05, seed 77",
+        "config": {"workload": f"{args.images} images {int(widths.min())}..{int(widths.max())} x "
+                               f"{int(heights.min())}..{int(heights.max())}, {args.iterations} iterations",
+                   "pixels": int(pixels), "time_tile": T, "tile": [bx, by * e], "elements_per_thread": e,
+                   "items": bt.items, "workgroups": bt.workgroups, "launches_per_step": bt.launches},
+        "batched_eager_ms": round(batched_eager * 1e3, 4),
+        "per_image_ms": round(per_image * 1e3, 4),
+        "per_image_graph_ms": round(per_image_graph * 1e3, 4),
+        "per_image_launches": int(len(ims) * bt.launches),
+        "speedup_vs_per_image": round(per_image / batched, 3),
+        "speedup_vs_per_image_graph": round(per_image_graph / batched, 3),
+    }
+    bt.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload.startswith("batch-"):
+        if args.T == DEFAULT_TILED["T"] and args.bx == DEFAULT_TILED["x"]:
+            args.T, args.bx, args.by, args.ept = DEFAULT_BATCH[args.workload]
+        bench_batch(args, args.workload.split("-", 1)[1])
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this engine's path is single-GPU; "

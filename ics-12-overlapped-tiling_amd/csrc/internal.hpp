@@ -28,7 +28,16 @@ struct LaunchGeom {
   int32_t phase_limit;
   int32_t variant;
   float coef[4];
+  int32_t wave_rows;     // 3D: footprint rows of the wave-pipelined kernel, 0 = LDS kernel
+  int32_t reserved;
 };
+
+// Register budget (3*T*rows per lane) of the wave-pipelined 3D kernels.
+constexpr int kW3RegBudget = 192;
+
+// Footprint rows of the wave-pipelined 3D kernel for this tile, or 0 when the
+// tile does not fill a wavefront (x tile + 2T == 64) or has no compiled row count.
+int k3w_rows(int T, const int32_t tile[3]);
 
 // Per-launch field pointers (up to 3 rw fields, 2 ro fields).
 struct FieldPtrs {
@@ -54,6 +63,10 @@ ot_status launch_3d_j3d(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs
 ot_status launch_3d_g3d(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3d_p3d7(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3d_p3d27(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3dw_j3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3dw_g3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3dw_p3d7(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3dw_p3d27(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 
 // Raise the dynamic-LDS limit of a kernel that needs more than 64 KiB.
 ot_status prepare_lds(const void* kernel, int lds_bytes);

@@ -139,25 +139,29 @@ __global__ void __launch_bounds__(MAXT) k3d(LaunchGeom g, FieldPtrs f) {
   const bool do_store = g.phase_limit == 0;
   const int s_end = (g.phase_limit == 2) ? 1 : T;
 
-  float pf[E], pfr[E];
-  auto fetch = [&](int k) {
-    if (do_loads && k >= 0 && k < g.n[2]) {
-      const int soff = (int)((uint32_t)(k - (z0 - T)) * plane_bytes);
+  // Level-0 planes are prefetched PD iterations ahead into a register ring.
+  // Every load and store is issued unconditionally — a plane outside the
+  // domain (or no output plane) uses the out-of-range lane offset, which reads
+  // 0 and drops stores without touching memory — so the number of memory
+  // operations between a load and its use is fixed and the wait before the use
+  // only covers that load, not the stores issued after it.
+  constexpr int PD = 2;
+  static_assert(U % PD == 0, "prefetch ring must divide the unroll");
+  float pf[PD][E], pfr[PD][E];
+  auto fetch = [&](int k, auto slotc) {
+    constexpr int SL = decltype(slotc)::value;
+    const bool live = do_loads && k >= 0 && k < g.n[2] && k <= kload_end;
+    const int soff = (int)((uint32_t)(live ? k - (z0 - T) : 0) * plane_bytes);
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        pf[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)voff[i], soff, 0));
-        if constexpr (RHS)
-          pfr[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rro, (int)voff[i], soff, 0));
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < E; ++i) {
-        pf[i] = 0.0f;
-        pfr[i] = 0.0f;
-      }
+    for (int i = 0; i < E; ++i) {
+      const int vo = (int)(live ? voff[i] : kOOB);
+      pf[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, vo, soff, 0));
+      if constexpr (RHS)
+        pfr[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rro, vo, soff, 0));
     }
   };
-  fetch(kstart);
+  fetch(kstart, IC<0>{});
+  fetch(kstart + 1, IC<1>{});
 
   // Stage T's plane is stored at the top of the next iteration, next to the
   // prefetch: on gfx9 stores count in vmcnt, so a store issued at the end of an
@@ -166,12 +170,12 @@ __global__ void __launch_bounds__(MAXT) k3d(LaunchGeom g, FieldPtrs f) {
   float outv[E];
   int out_p = -1;
   auto flush = [&]() {
-    if (out_p >= 0) {
-      const int soff = (int)((uint32_t)(out_p - (z0 - T)) * plane_bytes);
+    const bool live = out_p >= 0;
+    const int soff = (int)((uint32_t)(live ? out_p - (z0 - T) : 0) * plane_bytes);
 #pragma unroll
-      for (int i = 0; i < E; ++i)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[i]), rdst, (int)soff_out[i], soff, 0);
-    }
+    for (int i = 0; i < E; ++i)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[i]), rdst,
+                                            (int)(live ? soff_out[i] : kOOB), soff, 0);
   };
 
   // One iteration at compile-time phase PH (= (k - kstart) mod U).
@@ -190,19 +194,20 @@ __global__ void __launch_bounds__(MAXT) k3d(LaunchGeom g, FieldPtrs f) {
     __syncthreads();
 
     // ---- level 0: plane k (prefetched); store the previous output plane, then
-    // start fetching plane k+1
+    // start fetching plane k+PD into the ring slot just consumed
+    constexpr int PS = PH % PD;
     if (k <= kload_end) {
       float* D = lds + SW * plane + my;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        q[0][QN][i] = pf[i];
-        D[i * FX] = pf[i];
-        if constexpr (RHS) fq[0][i] = pfr[i];
+        q[0][QN][i] = pf[PS][i];
+        D[i * FX] = pf[PS][i];
+        if constexpr (RHS) fq[0][i] = pfr[PS][i];
       }
     }
     flush();
     out_p = -1;
-    if (k + 1 <= kload_end) fetch(k + 1);
+    fetch(k + PD, IC<PS>{});
 
     if constexpr (L == 1) {
       // ---- 7-point stages. Every stage reads only LDS planes written in an
@@ -334,7 +339,7 @@ ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool p
   // buffer descriptors: in-plane byte offsets must stay below the 2^31
   // out-of-range marker and a chunk's plane offsets must fit 32 bits
   const int64_t plane_bytes = g.sz * 4;
-  if ((int64_t)(g.tile[2] + 2 * T + 1) * plane_bytes >= (1LL << 31))
+  if ((int64_t)(g.tile[2] + 2 * T + 3) * plane_bytes >= (1LL << 31))
     return set_error(OT_EUNSUPPORTED, "3D plane of %lld bytes too large for a z-chunk of %d planes",
                      (long long)plane_bytes, g.tile[2]);
   const bool small = bx * by <= 512;

@@ -1,0 +1,325 @@
+// k3w_impl.hpp — wave-pipelined overlapped time-tiled 3D kernels.
+//
+// A second kernel family for the 3D stencils (jacobi-3d.cpp:120-432,
+// poisson-3d.cpp:120-398, gradient-3d.cpp:120-434) with no LDS and no
+// barriers: every wavefront is an independent tile.
+//   * The 64 lanes of a wavefront are 64 consecutive x columns of the
+//     footprint; every lane holds a whole column strip of FR footprint rows in
+//     registers. x-neighbours come from the adjacent lanes through DPP
+//     wave-shift operands (wave_shr:1 / wave_shl:1, folded into the adds),
+//     y-neighbours are the lane's own neighbouring rows.
+//   * The wave streams its z-chunk plane by plane; stage s (1..T) computes time
+//     level s of plane k - s (k - 2s for the 27-point stencil) from a 3-deep
+//     register queue of level s-1 planes. Stage s only evaluates the rows
+//     s..FR-1-s still needed by later stages, so the redundant halo work is
+//     the x-halo of 2T lanes plus a shrinking y-halo.
+//   * Output tile = (64 - 2T) x (FR - 2T) points per plane; the level-0 plane
+//     is prefetched PD planes ahead through buffer loads whose out-of-range
+//     lane offset reads 0 outside the domain; results are stored through
+//     buffer stores one iteration later (so load waits never wait for stores).
+// Out-of-domain points are forced to +0 at every level (Dirichlet 0).
+#pragma once
+#include "internal.hpp"
+#include "device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ot {
+namespace k3w {
+
+using namespace dev;
+
+template <int V>
+struct IC {
+  static constexpr int value = V;
+};
+
+// lane i receives lane i-1's value (lane 0 receives 0)
+__device__ __forceinline__ float from_left(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+// lane i receives lane i+1's value (lane 63 receives 0)
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
+template <int ST>
+struct KW {
+  static constexpr int L = (ST == OT_P3D27) ? 2 : 1;
+  static constexpr bool RHS = (ST == OT_P3D7);
+  static constexpr int NQ = (L == 2) ? 4 : 3;  // register planes kept per level
+  static constexpr int U = (L == 2) ? 4 : 6;   // z-loop unroll (multiple of NQ and of PD)
+  static constexpr int PD = (L == 2) ? 2 : 2;  // planes prefetched ahead
+};
+
+// Waves per workgroup (the workgroup only bundles independent waves).
+constexpr int kWaves = 4;
+
+template <int ST, int T, int FR>
+__global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
+  constexpr int L = KW<ST>::L;
+  constexpr bool RHS = KW<ST>::RHS;
+  constexpr int NQ = KW<ST>::NQ;
+  constexpr int U = KW<ST>::U;
+  constexpr int PD = KW<ST>::PD;
+  static_assert(U % PD == 0 && U % NQ == 0, "unroll must cycle the prefetch ring and the queue");
+  static_assert(FR > 2 * T, "footprint rows must exceed the halo");
+  const int lane = threadIdx.x & 63;
+  const int nblk = (int)gridDim.x;
+  // wave-uniform by construction; readfirstlane tells the compiler so, which
+  // keeps the tile origin and the buffer descriptors in SGPRs
+  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, nblk) * kWaves +
+                                                 (int)(threadIdx.x >> 6));
+  const int nw = g.grid[0] * g.grid[1] * g.grid[2];
+  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
+  const int bxi = wid % g.grid[0];
+  const int byi = (wid / g.grid[0]) % g.grid[1];
+  const int bzi = wid / (g.grid[0] * g.grid[1]);
+  // tile (TX, TY) with TX + 2T <= 64 lanes and TY + 2T <= FR rows; a plan
+  // uses TX = 64 - 2T and FR = TY + 2T, a shortened last launch (smaller T)
+  // leaves the spare lanes and rows as dead halo
+  const int TX = g.tile[0], TY = g.tile[1], TZ = g.tile[2];
+  const int x0 = bxi * TX - T, y0 = byi * TY - T, z0 = bzi * TZ;
+  const int gx = x0 + lane;
+  const bool in_x = gx >= 0 && gx < g.n[0];
+  const bool out_x = lane >= T && lane < T + TX;
+
+  const int64_t zbase = (int64_t)(z0 - T) * g.sz;
+  const uint32_t plane_bytes = (uint32_t)(g.sz * 4);
+  // chunk = the TZ + 2T planes this wave loads (+3 spare); the host keeps it
+  // under 1 GiB, so no offset sum below wraps 32 bits
+  const uint32_t range = (uint32_t)(TZ + 2 * T + 3) * plane_bytes;
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(f.src[0] + g.origin + zbase), (short)0, (int)range, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rdst =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(f.dst[0] + g.origin + zbase), (short)0, (int)range, 0x00020000);
+  __amdgpu_buffer_rsrc_t rro = rsrc;
+  if constexpr (RHS)
+    rro = __builtin_amdgcn_make_buffer_rsrc((void*)(f.ro[0] + g.origin + zbase), (short)0, (int)range, 0x00020000);
+
+  // Byte offset of row i of plane k = lane term + (row term + plane term).
+  // A lane outside the domain has lane term = range, a row or plane outside it
+  // row term = range: the offset then reaches the descriptor's range, so the
+  // load returns 0 and the store is dropped. The row terms are
+  // wave-uniform and live in SGPRs; only the two lane terms take VGPRs.
+  const uint32_t lane_ld = in_x ? (uint32_t)gx * 4u : range;
+  const uint32_t lane_st = (in_x && out_x) ? (uint32_t)gx * 4u : range;
+  uint32_t row_ld[FR], row_st[FR], mxy[FR];
+#pragma unroll
+  for (int i = 0; i < FR; ++i) {
+    const int gy = y0 + i;
+    const bool row_in = gy >= 0 && gy < g.n[1];
+    const uint32_t o = row_in ? (uint32_t)((int64_t)gy * g.sy * 4) : 0u;
+    row_ld[i] = row_in ? o : range;
+    row_st[i] = (row_in && i >= T && i < T + TY) ? o : range;
+    mxy[i] = (in_x && row_in) ? 0xFFFFFFFFu : 0u;
+  }
+  if (g.phase_limit == 1) return;
+
+  float q[T][NQ][FR];
+  float fq[T + 1][FR];
+#pragma unroll
+  for (int l = 0; l < T; ++l)
+#pragma unroll
+    for (int a = 0; a < NQ; ++a)
+#pragma unroll
+      for (int i = 0; i < FR; ++i) q[l][a][i] = 0.0f;
+#pragma unroll
+  for (int j = 0; j <= T; ++j)
+#pragma unroll
+    for (int i = 0; i < FR; ++i) fq[j][i] = 0.0f;
+
+  const int kstart = z0 - T;
+  const int kload_end = z0 + TZ - 1 + T;
+  const int kend = z0 + TZ - 1 + L * T;
+  const int zout_end = min(z0 + TZ, g.n[2]);
+  const bool do_loads = g.phase_limit != 3;
+  const bool do_store = g.phase_limit == 0;
+  const int s_end = (g.phase_limit == 2) ? 1 : T;
+
+  float pf[PD][FR], pfr[PD][FR];
+  auto fetch = [&](int k, auto slotc) {
+    constexpr int SL = decltype(slotc)::value;
+    const bool live = do_loads && k >= 0 && k < g.n[2] && k <= kload_end;
+    const uint32_t poff = (uint32_t)(live ? k - (z0 - T) : 0) * plane_bytes;
+#pragma unroll
+    for (int i = 0; i < FR; ++i) {
+      // the plane term changes every iteration, which keeps the compiler from
+      // hoisting FR per-row sums into VGPRs
+      const int vo = (int)(lane_ld + (live ? row_ld[i] + poff : range));
+      pf[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, vo, 0, 0));
+      if constexpr (RHS)
+        pfr[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rro, vo, 0, 0));
+    }
+  };
+  fetch(kstart, IC<0>{});
+  if constexpr (PD > 1) fetch(kstart + 1, IC<1>{});
+  if constexpr (PD > 2) fetch(kstart + 2, IC<2>{});
+  if constexpr (PD > 3) fetch(kstart + 3, IC<3>{});
+  if constexpr (PD > 4) fetch(kstart + 4, IC<4>{});
+  if constexpr (PD > 5) fetch(kstart + 5, IC<5>{});
+
+  float outv[FR];
+#pragma unroll
+  for (int i = 0; i < FR; ++i) outv[i] = 0.0f;
+  int out_p = -1;
+  auto flush = [&]() {
+    const bool live = out_p >= 0;
+    const uint32_t poff = (uint32_t)(live ? out_p - (z0 - T) : 0) * plane_bytes;
+#pragma unroll
+    for (int i = T; i < FR - T; ++i)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[i]), rdst,
+                                            (int)(lane_st + (live ? row_st[i] + poff : range)), 0, 0);
+  };
+
+  auto iter = [&](auto phc, const int k) {
+    constexpr int PH = decltype(phc)::value;
+    constexpr int PS = PH % PD;
+    // register queue slots: the plane produced at iteration j sits in slot j mod NQ
+    constexpr int QN = PH % NQ;
+    constexpr int Q1 = (PH + NQ - 1) % NQ;
+    constexpr int Q2 = (PH + NQ - 2) % NQ;
+    constexpr int Q3 = (PH + NQ - 3) % NQ;
+    if constexpr (RHS) {
+#pragma unroll
+      for (int j = T; j >= 1; --j)
+#pragma unroll
+        for (int i = 0; i < FR; ++i) fq[j][i] = fq[j - 1][i];
+    }
+#pragma unroll
+    for (int i = 0; i < FR; ++i) {
+      q[0][QN][i] = pf[PS][i];
+      if constexpr (RHS) fq[0][i] = pfr[PS][i];
+    }
+    flush();
+    out_p = -1;
+    fetch(k + PD, IC<PS>{});
+
+#pragma unroll
+    for (int s = 1; s <= T; ++s) {
+      if (s > s_end) break;
+      const int p = k - L * s;
+      const int lo = z0 - (T - s), hi = z0 + TZ - 1 + (T - s);
+      if (p < lo || p > hi) continue;
+      if (p < 0 || p >= g.n[2]) {
+        // plane outside the domain: level s is 0 there (wave-uniform branch,
+        // so the per-point mask below only carries the x/y validity)
+        if (s < T) {
+#pragma unroll
+          for (int i = s; i < FR - s; ++i) q[s][QN][i] = 0.0f;
+        }
+        continue;
+      }
+#pragma unroll
+      for (int i = s; i < FR - s; ++i) {
+        float v;
+        if constexpr (L == 1) {
+          const float c = q[s - 1][Q1][i];
+          const float zm = q[s - 1][Q2][i];
+          const float zp = q[s - 1][QN][i];
+          const float ym = q[s - 1][Q1][i - 1];
+          const float yp = q[s - 1][Q1][i + 1];
+          const float xm = from_left(c), xp = from_right(c);
+          if constexpr (ST == OT_J3D) v = j3d(xm, xp, ym, yp, zm, zp, c);
+          else if constexpr (ST == OT_G3D) v = g3d(xm, xp, ym, yp, zm, zp, c);
+          else v = p3d7(xm, xp, ym, yp, zm, zp, fq[s][i]);
+        } else {
+          // poisson-3d.cpp:282-293: planes p-1, p, p+1 of level s-1 were
+          // produced 3, 2 and 1 iterations ago
+          const float* Pz[3] = {q[s - 1][Q3], q[s - 1][Q2], q[s - 1][Q1]};
+          float sum = 0.0f;
+          bool first = true;
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+              for (int dz = -1; dz <= 1; ++dz) {
+                if (dx == 0 && dy == 0 && dz == 0) continue;
+                const float t0 = Pz[dz + 1][i + dy];
+                const float t = dx < 0 ? from_left(t0) : (dx > 0 ? from_right(t0) : t0);
+                sum = first ? t : sum + t;
+                first = false;
+              }
+          v = 26.0f * Pz[1][i] - sum;
+        }
+        const float nv = __uint_as_float(__float_as_uint(v) & mxy[i]);
+        if (s < T) q[s][QN][i] = nv;
+        else outv[i] = nv;
+      }
+      if (s == T && do_store && p >= z0 && p < zout_end) out_p = p;
+    }
+  };
+
+  int k = kstart;
+  for (; k + U - 1 <= kend; k += U) {
+    iter(IC<0>{}, k);
+    iter(IC<1>{}, k + 1);
+    iter(IC<2>{}, k + 2);
+    iter(IC<3>{}, k + 3);
+    if constexpr (U > 4) {
+      iter(IC<4>{}, k + 4);
+      iter(IC<5>{}, k + 5);
+    }
+  }
+  if (k <= kend) iter(IC<0>{}, k);
+  if (k + 1 <= kend) iter(IC<1>{}, k + 1);
+  if (k + 2 <= kend) iter(IC<2>{}, k + 2);
+  if constexpr (U > 4) {
+    if (k + 3 <= kend) iter(IC<3>{}, k + 3);
+    if (k + 4 <= kend) iter(IC<4>{}, k + 4);
+  }
+  flush();
+}
+
+template <int ST, int T, int FR>
+ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  const int64_t plane_bytes = g.sz * 4;
+  if ((int64_t)(g.tile[2] + 2 * T + 3) * plane_bytes >= (1LL << 30))
+    return set_error(OT_EUNSUPPORTED, "3D plane of %lld bytes too large for a z-chunk of %d planes",
+                     (long long)plane_bytes, g.tile[2]);
+  if (prepare) return prepare_lds((const void*)&k3w<ST, T, FR>, 0);
+  const int nw = g.grid[0] * g.grid[1] * g.grid[2];
+  const int nblk = (nw + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL((k3w<ST, T, FR>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return set_error(OT_EHIP, "3D wave kernel launch failed: %s", hipGetErrorString(err));
+  return OT_OK;
+}
+
+// Footprint row counts compiled per time tile (registers: 3*T*FR per lane);
+// k3w_rows() in runtime.hip applies the same rule.
+template <int ST, int T>
+ot_status launch_fr(int fr, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  switch (fr) {
+#define FRC(V)                                                      \
+  case V:                                                           \
+    if constexpr (V > 2 * T && 3 * T * V <= kW3RegBudget) return launch_t<ST, T, V>(g, f, stream, prepare); \
+    break;
+    FRC(8)
+    FRC(12)
+    FRC(16)
+    FRC(20)
+    FRC(24)
+    FRC(32)
+#undef FRC
+    default:
+      break;
+  }
+  return set_error(OT_EUNSUPPORTED, "wave-pipelined 3D kernel not compiled for T=%d with %d footprint rows", T, fr);
+}
+
+template <int ST>
+ot_status launch_st(int t_eff, int fr, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  switch (t_eff) {
+    case 1: return launch_fr<ST, 1>(fr, g, f, stream, prepare);
+    case 2: return launch_fr<ST, 2>(fr, g, f, stream, prepare);
+    case 3: return launch_fr<ST, 3>(fr, g, f, stream, prepare);
+    case 4: return launch_fr<ST, 4>(fr, g, f, stream, prepare);
+    case 5: return launch_fr<ST, 5>(fr, g, f, stream, prepare);
+    default: return set_error(OT_EUNSUPPORTED, "wave-pipelined 3D kernels are compiled for T = 1..5 (got %d)", t_eff);
+  }
+}
+
+}  // namespace k3w
+}  // namespace ot

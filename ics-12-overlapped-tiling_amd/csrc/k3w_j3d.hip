@@ -1,0 +1,9 @@
+// k3w_j3d.hip — instantiations of the wave-pipelined 3D kernel for OT_J3D
+// (own translation unit so the build compiles them in parallel).
+#include "k3w_impl.hpp"
+
+namespace ot {
+ot_status launch_3dw_j3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  return k3w::launch_st<OT_J3D>(t_eff, rows, g, f, stream, prepare);
+}
+}  // namespace ot

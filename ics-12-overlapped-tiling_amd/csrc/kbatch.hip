@@ -127,6 +127,9 @@ struct ot_batch {
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
   const float* graph_key[8] = {};
+  int graph_per_image = 0;
+  std::vector<ot_image> host_images;
+  bool per_image_mode = false;  // set for the duration of one ot_batch_run
 };
 
 static ot_status make_shape(int stencil, int t_eff, int ept, const int32_t tile[3], LaunchShape& s, int n_items) {
@@ -148,8 +151,49 @@ static ot_status make_shape(int stencil, int t_eff, int ept, const int32_t tile[
   return OT_OK;
 }
 
+// Baseline: every image advanced by its own per-grid launches.
+static ot_status per_image_enqueue(ot_batch* b, float* const* rw_a, float* const* rw_b, const float* const* ro,
+                                   hipStream_t s) {
+  const int T = b->p.time_tile;
+  for (int l = 0; l < b->launches; ++l) {
+    const int t_eff = std::min(T, b->p.time_steps - l * T);
+    for (int i = 0; i < b->n_images; ++i) {
+      const ot_image& im = b->host_images[i];
+      LaunchGeom g;
+      std::memset(&g, 0, sizeof(g));
+      g.n[0] = im.width;
+      g.n[1] = im.height;
+      g.n[2] = 1;
+      g.tile[0] = b->p.block_x;
+      g.tile[1] = b->p.block_y * b->p.elems_per_thread;
+      g.tile[2] = 1;
+      g.grid[0] = (int)ceil_div(im.width, g.tile[0]);
+      g.grid[1] = (int)ceil_div(im.height, g.tile[1]);
+      g.grid[2] = 1;
+      g.sy = im.pitch;
+      g.sz = (int64_t)im.pitch * im.height;
+      g.origin = im.offset;
+      g.phase_limit = b->p.phase_limit;
+      g.variant = b->p.variant;
+      for (int c = 0; c < 4; ++c) g.coef[c] = b->p.coef[c];
+      FieldPtrs f;
+      std::memset(&f, 0, sizeof(f));
+      const bool even = (l % 2) == 0;
+      for (int k = 0; k < b->num_rw; ++k) {
+        f.src[k] = even ? rw_a[k] : rw_b[k];
+        f.dst[k] = even ? rw_b[k] : rw_a[k];
+      }
+      for (int k = 0; k < b->num_ro; ++k) f.ro[k] = ro[k];
+      const ot_status st = launch_2d(b->p.stencil, t_eff, b->p.elems_per_thread, g, f, s, false);
+      if (st != OT_OK) return st;
+    }
+  }
+  return OT_OK;
+}
+
 static ot_status batch_enqueue(ot_batch* b, float* const* rw_a, float* const* rw_b, const float* const* ro,
                                hipStream_t s) {
+  if (b->per_image_mode) return per_image_enqueue(b, rw_a, rw_b, ro, s);
   hipLaunchKernelGGL(zero_counters, dim3((b->launches + 255) / 256), dim3(256), 0, s, b->d_counters, b->launches);
   OT_HIP(hipGetLastError());
   const int T = b->p.time_tile;
@@ -240,10 +284,21 @@ ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_im
   b->launches = (int)ceil_div(p.time_steps, p.time_tile);
   b->num_rw = kTraits[p.stencil].num_rw;
   b->num_ro = kTraits[p.stencil].num_ro;
+  b->host_images.assign(images, images + n_images);
   ot_status st = make_shape(p.stencil, p.time_tile, p.elems_per_thread, tile, b->full, b->n_items);
   const int last = p.time_steps % p.time_tile;
   if (st == OT_OK) st = last ? make_shape(p.stencil, last, p.elems_per_thread, tile, b->last, b->n_items) : OT_OK;
   if (st == OT_OK && last == 0) b->last = b->full;
+  // the per-image baseline launches the per-grid kernels of the same tile
+  if (st == OT_OK) {
+    LaunchGeom g;
+    std::memset(&g, 0, sizeof(g));
+    for (int a = 0; a < 3; ++a) g.tile[a] = tile[a];
+    FieldPtrs f;
+    std::memset(&f, 0, sizeof(f));
+    st = launch_2d(p.stencil, p.time_tile, p.elems_per_thread, g, f, nullptr, true);
+    if (st == OT_OK && last) st = launch_2d(p.stencil, last, p.elems_per_thread, g, f, nullptr, true);
+  }
   hipError_t e = hipSuccess;
   if (st == OT_OK) {
     e = hipMalloc(&b->d_images, sizeof(BImage) * n_images);
@@ -279,6 +334,7 @@ ot_status ot_batch_run(ot_batch* b, float* const* rw_a, float* const* rw_b, cons
     if (!ro || !ro[k]) return set_error(OT_EINVAL, "null read-only arena %d", k);
   if (mode & OT_RUN_TIME_LAUNCHES) return set_error(OT_EINVAL, "per-launch timing is not offered for batches");
   const bool use_graph = (mode & OT_RUN_GRAPH) != 0;
+  b->per_image_mode = (mode & OT_RUN_PER_IMAGE) != 0;
   hipStream_t s = (hipStream_t)stream;
   if (!s && use_graph) {
     if (!b->own_stream) OT_HIP(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
@@ -303,7 +359,8 @@ ot_status ot_batch_run(ot_batch* b, float* const* rw_a, float* const* rw_b, cons
       key[nk++] = rw_b[k];
     }
     for (int k = 0; k < b->num_ro; ++k) key[nk++] = ro[k];
-    if (!(b->exec && b->graph_stream == s && std::memcmp(key, b->graph_key, sizeof(key)) == 0)) {
+    if (!(b->exec && b->graph_stream == s && b->graph_per_image == (int)b->per_image_mode &&
+          std::memcmp(key, b->graph_key, sizeof(key)) == 0)) {
       if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -321,6 +378,7 @@ ot_status ot_batch_run(ot_batch* b, float* const* rw_a, float* const* rw_b, cons
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return set_error(OT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
       b->graph_stream = s;
+      b->graph_per_image = (int)b->per_image_mode;
       std::memcpy(b->graph_key, key, sizeof(key));
     }
   }

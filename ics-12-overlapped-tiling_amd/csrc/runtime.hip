@@ -57,8 +57,26 @@ void shape_3d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* 
   *lds = (T * NS * (*by) * ept * FX + 2 * (FX + 1)) * (int)sizeof(float);
 }
 
+int k3w_rows(int T, const int32_t tile[3]) {
+  if (tile[0] + 2 * T != 64) return 0;
+  const int fr = tile[1] + 2 * T;
+  static const int kRows[] = {8, 12, 16, 20, 24, 32};
+  for (int r : kRows)
+    if (r == fr && fr > 2 * T && 3 * T * fr <= kW3RegBudget) return fr;
+  return 0;
+}
+
 ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
                     void* stream, bool prepare) {
+  if (g.wave_rows > 0) {
+    switch (stencil) {
+      case OT_J3D: return launch_3dw_j3d(t_eff, g.wave_rows, g, f, stream, prepare);
+      case OT_G3D: return launch_3dw_g3d(t_eff, g.wave_rows, g, f, stream, prepare);
+      case OT_P3D7: return launch_3dw_p3d7(t_eff, g.wave_rows, g, f, stream, prepare);
+      case OT_P3D27: return launch_3dw_p3d27(t_eff, g.wave_rows, g, f, stream, prepare);
+      default: return set_error(OT_EINVAL, "stencil %d is not a 3D stencil", stencil);
+    }
+  }
   switch (stencil) {
     case OT_J3D: return launch_3d_j3d(t_eff, ept, g, f, stream, prepare);
     case OT_G3D: return launch_3d_g3d(t_eff, ept, g, f, stream, prepare);
@@ -75,8 +93,15 @@ ot_status kernel_shape(const ot_params& p, ot_derived& d) {
     shape_1d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &threads, &lds);
   } else {
     int bx = 0, by = 0;
-    if (dims == 2) shape_2d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
-    else shape_3d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
+    if (dims == 2) {
+      shape_2d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
+    } else if (k3w_rows(p.time_tile, d.tile) > 0) {
+      bx = 256;  // four independent wavefronts per workgroup, no LDS
+      by = 1;
+      lds = 0;
+    } else {
+      shape_3d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
+    }
     threads = bx * by;
   }
   d.threads_per_block = threads;
@@ -199,6 +224,7 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   g.phase_limit = params->phase_limit;
   g.variant = params->variant;
   for (int i = 0; i < 4; ++i) g.coef[i] = params->coef[i];
+  g.wave_rows = (d.dims == 3) ? k3w_rows(params->time_tile, d.tile) : 0;
   // Check the instantiations this plan will launch (full T and the shortened
   // last launch) exist, and raise their LDS limits now.
   FieldPtrs f;

@@ -13,6 +13,8 @@ import numpy as np
 import otlib
 from otlib import DeviceBuffer, Image, Params, Timing, _ptr_array, check, lib
 
+RUN_PER_IMAGE = 4
+
 
 def layout(widths, heights):
     """(images ctypes array, arena elements) for the given extents."""
@@ -52,12 +54,15 @@ class Batch:
         check(lib().ot_batch_info(h, C.byref(items), C.byref(wgs), C.byref(launches)))
         self.items, self.workgroups, self.launches = items.value, wgs.value, launches.value
 
-    def run(self, rw_a, rw_b, ro=(), use_graph=False, stream=None) -> Timing:
+    def run(self, rw_a, rw_b, ro=(), use_graph=False, per_image=False, stream=None) -> Timing:
+        """per_image=True: the baseline of one per-grid launch per image and
+        time tile instead of the work-queue kernel."""
         a = _ptr_array([b.ptr for b in rw_a])
         b = _ptr_array([x.ptr for x in rw_b])
         r = _ptr_array([x.ptr for x in ro]) if ro else None
         tm = Timing()
-        check(lib().ot_batch_run(self.handle, a, b, r, stream, otlib.RUN_GRAPH if use_graph else 0, C.byref(tm)))
+        mode = (otlib.RUN_GRAPH if use_graph else 0) | (RUN_PER_IMAGE if per_image else 0)
+        check(lib().ot_batch_run(self.handle, a, b, r, stream, mode, C.byref(tm)))
         return tm
 
     def close(self):
@@ -72,13 +77,13 @@ class Batch:
             pass
 
 
-def run_batch_host(params: Params, images, rw_arenas, ro_arenas=(), use_graph=False):
+def run_batch_host(params: Params, images, rw_arenas, ro_arenas=(), use_graph=False, per_image=False):
     """Upload arenas, run the batch, download the rw arenas."""
     bt = Batch(params, images)
     a = [DeviceBuffer.from_array(x) for x in rw_arenas]
     b = [DeviceBuffer.from_array(x) for x in rw_arenas]
     ro = [DeviceBuffer.from_array(x) for x in ro_arenas]
-    tm = bt.run(a, b, ro, use_graph=use_graph)
+    tm = bt.run(a, b, ro, use_graph=use_graph, per_image=per_image)
     src = a if tm.final_buffer == 0 else b
     out = [buf.download(x.shape) for buf, x in zip(src, rw_arenas)]
     for buf in a + b + ro:

@@ -157,8 +157,10 @@ typedef struct ot_timing {
 
 /* ot_run mode bits */
 enum {
-  OT_RUN_GRAPH = 1,        /* capture the loop into a hipGraph once, replay it */
-  OT_RUN_TIME_LAUNCHES = 2 /* bracket every launch with HIP events (eager only) */
+  OT_RUN_GRAPH = 1,         /* capture the loop into a hipGraph once, replay it */
+  OT_RUN_TIME_LAUNCHES = 2, /* bracket every launch with HIP events (eager only) */
+  OT_RUN_PER_IMAGE = 4      /* ot_batch_run: one per-grid launch per image and time
+                               tile instead of the work-queue kernel (baseline) */
 };
 
 typedef struct ot_plan ot_plan;

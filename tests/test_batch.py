@@ -51,8 +51,11 @@ def _batch_case(st, T, bx, by, e, steps, seed, coef=(0, 0, 0, 0)):
     p = otlib.make_params(st, steps=steps, T=T, x=bx, y=by, e=e, coef=list(coef))
     out, tm = otbatch.run_batch_host(p, ims, rw, ro)
     out_g, _ = otbatch.run_batch_host(p, ims, rw, ro, use_graph=True)
-    for a, b in zip(out, out_g):
+    out_pi, _ = otbatch.run_batch_host(p, ims, rw, ro, per_image=True)
+    for a, b, c in zip(out, out_g, out_pi):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        for im in ims:  # padding columns are not written by the per-image launches either
+            assert np.array_equal(otbatch.image_view(a, im).view(np.uint32), otbatch.image_view(c, im).view(np.uint32))
     for im in ims:
         rw_i = [otbatch.image_view(x, im)[None].copy() for x in rw]
         ro_i = [otbatch.image_view(x, im)[None].copy() for x in ro]

@@ -56,6 +56,17 @@ CASES_3D = [
     (otlib.P3D7, (36, 30, 29), 7, 4, (16, 16, 8), 1),
     (otlib.P3D27, (36, 30, 29), 5, 2, (16, 8, 8), 1),
     (otlib.P3D27, (36, 30, 29), 5, 3, (8, 8, 4), 2),
+    # wave-pipelined kernels: x tile = 64 - 2T fills a wavefront
+    (otlib.J3D, (70, 37, 35), 7, 2, (60, 12, 8), 1),
+    (otlib.J3D, (130, 40, 33), 9, 3, (58, 10, 16), 1),
+    (otlib.J3D, (61, 29, 40), 11, 4, (56, 4, 16), 1),
+    (otlib.J3D, (64, 30, 31), 7, 1, (62, 30, 8), 1),
+    (otlib.G3D, (64, 30, 29), 6, 1, (62, 6, 8), 1),
+    (otlib.G3D, (90, 33, 20), 8, 3, (58, 2, 8), 1),
+    (otlib.P3D7, (70, 25, 29), 9, 4, (56, 4, 8), 1),
+    (otlib.P3D27, (70, 25, 29), 5, 2, (60, 8, 8), 1),
+    (otlib.P3D27, (66, 21, 17), 7, 3, (58, 6, 4), 1),
+    (otlib.J3D, (70, 30, 30), 12, 5, (54, 2, 8), 1),
 ]
 
 
