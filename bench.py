@@ -30,10 +30,29 @@ import numpy as np  # noqa: E402
 import otlib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Best tile configurations for Jacobi-3D 256^3 found by the engine's exhaustive
-# sweep (T, block x, block y rows, block z, elements per thread) and T=1.
-DEFAULT_TILED = dict(T=2, x=32, y=4, z=32, e=4)
-DEFAULT_T1 = dict(T=1, x=64, y=4, z=32, e=4)
+
+# Grid workloads: stencil, interior extents (nx, ny, nz), time steps, the best
+# tile configuration found by the engine's exhaustive sweep (T, block x, block y
+# rows, block z, elements per thread), the best T=1 configuration, and the
+# algorithmic HBM bytes per point per launch (every rw field read and written
+# once, every ro field read once).
+WORKLOADS = {
+    "jacobi3d": dict(stencil="jacobi-3d", n=(256, 256, 256), steps=32,
+                     tiled=dict(T=3, x=58, y=6, z=20, e=1), t1=dict(T=1, x=64, y=4, z=32, e=4),
+                     bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
+    "jacobi2d": dict(stencil="jacobi-2d", n=(1024, 1024, 1), steps=64,
+                     tiled=dict(T=4, x=16, y=16, z=1, e=1), t1=dict(T=1, x=64, y=4, z=1, e=4),
+                     bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
+    "fdtd2d": dict(stencil="fdtd-2d", n=(2048, 2048, 1), steps=100,
+                   tiled=dict(T=4, x=64, y=8, z=1, e=2), t1=dict(T=1, x=64, y=4, z=1, e=4),
+                   bytes_pp=24, data="Ex = Ey = 0, Hz Gaussian pulse (amplitude 1, sigma 8) at the centre"),
+    "gradient3d": dict(stencil="gradient-3d", n=(256, 256, 256), steps=64,
+                       tiled=dict(T=3, x=58, y=6, z=20, e=1), t1=dict(T=1, x=64, y=4, z=32, e=4),
+                       bytes_pp=8, data="sin(2 pi x/256) sin(2 pi y/256) sin(2 pi z/256) + U(-1e-3, 1e-3), seed 7"),
+    "poisson3d-rhs": dict(stencil="poisson-3d-rhs", n=(256, 256, 256), steps=32,
+                          tiled=dict(T=3, x=58, y=6, z=20, e=1), t1=dict(T=1, x=64, y=4, z=32, e=4),
+                          bytes_pp=12, data="u0 = 0, f uniform [-1, 1) from seed 11"),
+}
 # batch workloads: (T, block x, block y rows, elements per thread)
 DEFAULT_BATCH = {"batch-rician": (3, 32, 8, 4), "batch-tv": (2, 64, 4, 4)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jacobi3d_256_traffic.json")
@@ -44,24 +63,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed solves")
     ap.add_argument("--warmup", type=int, default=3, help="untimed solves")
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--time-steps", type=int, default=32)
-    ap.add_argument("--T", type=int, default=DEFAULT_TILED["T"])
-    ap.add_argument("--bx", type=int, default=DEFAULT_TILED["x"])
-    ap.add_argument("--by", type=int, default=DEFAULT_TILED["y"], help="block y in rows of ept")
-    ap.add_argument("--bz", type=int, default=DEFAULT_TILED["z"])
-    ap.add_argument("--ept", type=int, default=DEFAULT_TILED["e"])
+    ap.add_argument("--workload", default="jacobi3d", choices=list(WORKLOADS) + list(DEFAULT_BATCH))
+    ap.add_argument("--n", type=int, default=0, help="override the cube / square edge")
+    ap.add_argument("--time-steps", type=int, default=0, help="override the time-step count")
+    ap.add_argument("--T", type=int, default=0, help="override the time tile (with --bx/--by/--bz/--ept)")
+    ap.add_argument("--bx", type=int, default=0)
+    ap.add_argument("--by", type=int, default=0, help="block y in rows of ept")
+    ap.add_argument("--bz", type=int, default=0)
+    ap.add_argument("--ept", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-t1", action="store_true", help="skip the T=1 comparison")
-    ap.add_argument("--workload", default="jacobi3d",
-                    choices=["jacobi3d", "batch-rician", "batch-tv"],
-                    help="jacobi3d: the headline line; batch-*: 256 uneven images, batched persistent "
-                         "launches vs one launch per image and time tile")
     ap.add_argument("--images", type=int, default=256)
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run 5 device-to-device copies of MIB MiB (the reference kernel for scaling "
-                         "rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes) and exit")
+                    help="only time a device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -76,51 +91,80 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
 
 
-def solve_rate(cfg, n, steps_t, K, W, init):
+def make_inputs(w, d):
+    """Host arrays (layout of derived d) for the rw and ro fields of workload w."""
+    st = otlib.stencil_id(w["stencil"])
+    if st in (otlib.J3D, otlib.J2D):
+        return [otlib.fill_reference_uniform(d, 123456)], []
+    if st == otlib.FDTD2D:
+        z = otlib.empty_field(d)
+        return [z, z.copy(), otlib.fill_gaussian_pulse(d, 1.0, 8.0)], []
+    if st == otlib.G3D:
+        return [otlib.fill_sinusoid_noise(d, 7, 1e-3)], []
+    if st == otlib.P3D7:
+        return [otlib.empty_field(d)], [otlib.fill_uniform(d, 11, -1.0, 1.0)]
+    raise ValueError(w["stencil"])
+
+
+def plan_params(w, cfg, n3, steps):
+    st = otlib.stencil_id(w["stencil"])
+    return otlib.make_params(st, nx=n3[0], ny=n3[1], nz=n3[2], problem_size=n3[0], steps=steps, T=cfg["T"],
+                             x=cfg["x"], y=cfg["y"], z=cfg["z"], e=cfg["e"])
+
+
+def solve_rate(w, cfg, n3, steps, K, W):
     """Time K whole solves (graph replay) after W warm-ups. Returns
     (seconds per solve, average launch seconds from the launch-stream events,
-    launches per solve, eager per-launch average)."""
-    p = otlib.make_params(otlib.J3D, n=n, steps=steps_t, T=cfg["T"], x=cfg["x"], y=cfg["y"], z=cfg["z"],
-                          e=cfg["e"])
+    launches per solve, eager per-launch average, derived geometry)."""
+    p = plan_params(w, cfg, n3, steps)
     plan = otlib.Plan(p)
-    a = otlib.DeviceBuffer.from_array(init)
-    b = otlib.DeviceBuffer.from_array(init)
+    d = plan.derived
+    rw, ro = make_inputs(w, d)
+    A = [otlib.DeviceBuffer.from_array(x) for x in rw]
+    B = [otlib.DeviceBuffer.from_array(x) for x in rw]
+    R = [otlib.DeviceBuffer.from_array(x) for x in ro]
     for _ in range(W):
-        plan.run([a], [b], use_graph=True)
+        plan.run(A, B, R, use_graph=True)
     otlib.check(otlib.lib().ot_dev_sync())
     ev_total = 0.0
     t0 = time.perf_counter()
     for _ in range(K):
-        tm = plan.run([a], [b], use_graph=True)
+        tm = plan.run(A, B, R, use_graph=True)
         ev_total += tm.event_s
     otlib.check(otlib.lib().ot_dev_sync())
     t1 = time.perf_counter()
-    launches = plan.derived.launches
+    launches = d.launches
     # eager pass with one event pair per launch (cross-check of the launch time)
     eager = []
     for _ in range(max(2, min(K, 5))):
-        tm = plan.run([a], [b], time_launches=True)
+        tm = plan.run(A, B, R, time_launches=True)
         eager.append(tm.kernel_s / tm.launches)
     plan.close()
-    a.free()
-    b.free()
-    return (t1 - t0) / K, ev_total / (K * launches), launches, float(np.median(eager))
+    for buf in A + B + R:
+        buf.free()
+    return (t1 - t0) / K, ev_total / (K * launches), launches, float(np.median(eager)), d
 
 
-def cpu_baseline(n, steps_t):
+def cpu_baseline(w, n3, steps):
+    """The oracle (C restatement of the reference CPU loop) on the same
+    workload, single-threaded on the host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    rng = np.random.default_rng(12345)
-    u = rng.random((n, n, n), dtype=np.float32)
-    _, dt = oracle.run(otlib.J3D, [u], steps=steps_t, timed=True)
+    st = otlib.stencil_id(w["stencil"])
+    d = otlib.derive(plan_params(w, dict(T=1, x=16, y=4, z=4 if n3[2] > 1 else 1, e=1), n3, steps))
+    rw, ro = make_inputs(w, d)
+    sl = otlib.interior_slices(d)
+    _, dt = oracle.run(st, [x[sl].copy() for x in rw], [x[sl].copy() for x in ro], steps=steps, timed=True)
+    pts = float(n3[0]) * n3[1] * n3[2]
+    shape = "x".join(str(v) for v in n3 if v > 1)
     return {
-        "value": round(n ** 3 * steps_t / dt / 1e9, 4),
+        "value": round(pts * steps / dt / 1e9, 4),
         "unit": "GStencil/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"whole workload: Jacobi-3D {n}^3, {steps_t} time steps, oracle C restatement of the "
-                  f"reference CPU loop (jacobi-3d.cpp:729-746), gcc -O2 -ffp-contract=off, 1 thread of "
+        "sample": f"whole workload: {w['stencil']} {shape}, {steps} time steps, oracle C restatement of the "
+                  f"reference CPU loop, gcc -O2 -ffp-contract=off, 1 thread of "
                   f"{os.cpu_count()} ({platform.processor() or platform.machine()}), {dt:.2f} s",
     }
 
@@ -180,35 +224,52 @@ def bench_batch(args, stencil_name):
     print(json.dumps(out), flush=True)
 
 
+def kernel_name(w, d):
+    st = otlib.stencil_id(w["stencil"])
+    dims = otlib.stencil_info(st).dims
+    if dims == 3:
+        fam = "ot::k3w::k3w" if d.lds_bytes == 0 else "ot::k3::k3d"
+    else:
+        fam = {1: "ot::k1::k1d", 2: "ot::k2::k2d"}[dims]
+    return f"{fam}<{w['stencil']}>"
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
-    if args.workload.startswith("batch-"):
-        if args.T == DEFAULT_TILED["T"] and args.bx == DEFAULT_TILED["x"]:
-            args.T, args.bx, args.by, args.ept = DEFAULT_BATCH[args.workload]
-        bench_batch(args, args.workload.split("-", 1)[1])
-        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this engine's path is single-GPU; "
               f"see DESIGN.md for how the stencil would shard over several GPUs.", file=sys.stderr)
         sys.exit(2)
-    n, steps_t = args.n, args.time_steps
-    tiled = dict(T=args.T, x=args.bx, y=args.by, z=args.bz, e=args.ept)
-    d0 = otlib.derive(otlib.make_params(otlib.J3D, n=n, steps=steps_t, T=1, x=32, y=8, z=32, e=1))
-    init = otlib.fill_uniform(d0, 2024, 0.0, 1.0)
+    if args.workload.startswith("batch-"):
+        T, bx, by, e = DEFAULT_BATCH[args.workload]
+        if args.T:
+            T, bx, by, e = args.T, args.bx, args.by, args.ept
+        args.T, args.bx, args.by, args.ept = T, bx, by, e
+        bench_batch(args, args.workload.split("-", 1)[1])
+        return
+    w = WORKLOADS[args.workload]
+    n3 = w["n"]
+    if args.n:
+        n3 = tuple(args.n if v > 1 else 1 for v in n3)
+    steps_t = args.time_steps or w["steps"]
+    tiled = dict(w["tiled"])
+    if args.T:
+        tiled = dict(T=args.T, x=args.bx, y=args.by, z=args.bz or 1, e=args.ept or 1)
 
-    sec, launch_s, launches, eager_launch_s = solve_rate(tiled, n, steps_t, args.steps, args.warmup, init)
-    points = float(n) ** 3
+    sec, launch_s, launches, eager_launch_s, d = solve_rate(w, tiled, n3, steps_t, args.steps, args.warmup)
+    points = float(n3[0]) * n3[1] * n3[2]
     gst = points * steps_t / sec / 1e9
-    bytes_per_launch = 8.0 * points  # read + write every point once per launch
+    bytes_per_launch = float(w["bytes_pp"]) * points
     achieved = bytes_per_launch / launch_s / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE))
-        if tr.get("config") == {"n": n, **tiled}:
+        if tr.get("config") == {"workload": args.workload, "n": list(n3), **tiled}:
             traffic = tr.get("hbm_bytes_per_launch")
+    shape = "x".join(str(v) for v in n3 if v > 1)
     out = {
         "metric": "GStencil/s & %HBM roofline; learned tile pick vs exhaustive autotune",
         "value": round(gst, 2),
@@ -221,25 +282,26 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: uniform [0,1) interior from seed 2024, Dirichlet-0 boundary",
-        "config": {"workload": f"jacobi-3d 7-point {n}^3, {steps_t} time steps, time-tiled",
+        "data": "synthetic: " + w["data"],
+        "config": {"workload": f"{w['stencil']} {shape}, {steps_t} time steps, time-tiled",
                    "time_tile": tiled["T"], "tile": [tiled["x"], tiled["y"] * tiled["e"], tiled["z"]],
                    "elements_per_thread": tiled["e"], "launches_per_step": launches, "graph": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "ot::k3::k3d<J3D>", "avg_launch_us": round(launch_s * 1e6, 2),
+                     "kernel": kernel_name(w, d), "avg_launch_us": round(launch_s * 1e6, 2),
                      "eager_avg_launch_us": round(eager_launch_s * 1e6, 2),
                      "bytes_per_launch": bytes_per_launch},
     }
     if not args.no_t1:
-        sec1, launch1, l1, _ = solve_rate(DEFAULT_T1, n, steps_t, max(3, args.steps // 2), args.warmup, init)
+        t1 = w["t1"]
+        sec1, launch1, l1, _, _ = solve_rate(w, t1, n3, steps_t, max(3, args.steps // 2), args.warmup)
         g1 = points * steps_t / sec1 / 1e9
         out["t1"] = {"value": round(g1, 2), "ms_per_step": round(sec1 * 1e3, 4),
-                     "achieved_GBs": round(8.0 * points / launch1 / 1e9, 1),
-                     "tile": [DEFAULT_T1["x"], DEFAULT_T1["y"] * DEFAULT_T1["e"], DEFAULT_T1["z"]]}
+                     "achieved_GBs": round(bytes_per_launch / launch1 / 1e9, 1),
+                     "tile": [t1["x"], t1["y"] * t1["e"], t1["z"]]}
         out["speedup_vs_t1"] = round(gst / g1, 3)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, steps_t)
+        out["cpu_baseline"] = cpu_baseline(w, n3, steps_t)
     print(json.dumps(out), flush=True)
 
 
