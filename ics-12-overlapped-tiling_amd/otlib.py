@@ -86,10 +86,18 @@ class StencilInfo(C.Structure):
                 ("name", C.c_char_p)]
 
 
+class Arch(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "total_lds", "max_waves_per_cu", "max_threads_per_cu", "max_blocks_per_cu", "max_regs_per_cu",
+        "num_cu", "clock_hz", "c_load", "c_op", "b_gmem", "b_smem", "l_gmem", "l_smem", "launch_penalty",
+        "wave_size", "max_regs_per_thread", "l1_bytes", "sfu_cycles")]
+
+
 _FP = C.POINTER(C.c_float)
 _PP = C.POINTER(_FP)
+FEATURES = 24
 
-# (name, restype, argtypes) of every entry point declared in include/ot/ot.h
+# (name, restype, argtypes) of every entry point declared in include/ot/*.h
 SIGNATURES = [
     ("ot_version", C.c_char_p, []),
     ("ot_last_error", C.c_char_p, []),
@@ -122,6 +130,29 @@ SIGNATURES = [
     ("ot_fill_uniform", C.c_int, [_FP, C.POINTER(Derived), C.c_uint64, C.c_float, C.c_float]),
     ("ot_fill_sinusoid_noise", C.c_int, [_FP, C.POINTER(Derived), C.c_uint64, C.c_float]),
     ("ot_fill_gaussian_pulse", C.c_int, [_FP, C.POINTER(Derived), C.c_float, C.c_float]),
+    # ot_tune.h
+    ("ot_tune_space", C.c_int, [C.POINTER(Params), C.POINTER(Params), C.c_int32, C.POINTER(C.c_int32)]),
+    ("ot_tune_measure", C.c_int, [C.POINTER(Params), _PP, _PP, _PP, C.c_int32, C.c_int32, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("ot_tune_exhaustive", C.c_int, [C.POINTER(Params), C.c_char_p, C.c_int32, C.c_int32, C.c_double,
+                                     C.POINTER(Params), C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("ot_arch_default", C.c_int, [C.POINTER(Arch)]),
+    ("ot_model_analytic", C.c_int, [C.POINTER(Params), C.POINTER(Arch), C.c_int32, C.POINTER(C.c_double)]),
+    ("ot_features", C.c_int, [C.POINTER(Params), _FP]),
+    ("ot_model_create", C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("ot_model_destroy", None, [C.c_void_p]),
+    ("ot_model_param_count", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    ("ot_model_get_params", C.c_int, [C.c_void_p, _FP, C.c_int32]),
+    ("ot_model_set_params", C.c_int, [C.c_void_p, _FP, C.c_int32]),
+    ("ot_model_set_normalization", C.c_int, [C.c_void_p, _FP, _FP]),
+    ("ot_model_fit_normalization", C.c_int, [C.c_void_p, _FP, C.c_int32]),
+    ("ot_model_loss_grad", C.c_int, [C.c_void_p, _FP, _FP, C.c_int32, _FP, _FP]),
+    ("ot_model_train", C.c_int, [C.c_void_p, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                 _FP]),
+    ("ot_model_predict", C.c_int, [C.c_void_p, _FP, C.c_int32, _FP]),
+    ("ot_predict", C.c_int, [C.c_void_p, C.POINTER(Params), _FP]),
+    ("ot_model_save", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("ot_model_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
 ]
 
 _lib = None
