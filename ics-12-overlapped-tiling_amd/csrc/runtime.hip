@@ -261,6 +261,13 @@ void ot_plan_destroy(ot_plan* plan) {
   delete plan;
 }
 
+ot_status ot_plan_stream(ot_plan* plan, void** stream) {
+  if (!plan || !stream) return set_error(OT_EINVAL, "null argument");
+  if (!plan->own_stream) OT_HIP(hipStreamCreateWithFlags(&plan->own_stream, hipStreamNonBlocking));
+  *stream = (void*)plan->own_stream;
+  return OT_OK;
+}
+
 ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_dst,
                     const float* const* ro, int32_t base_time, void* stream) {
   if (!plan) return set_error(OT_EINVAL, "null plan");
@@ -290,8 +297,11 @@ ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const floa
   if (st != OT_OK) return st;
   const bool use_graph = (mode & OT_RUN_GRAPH) != 0;
   const bool time_launches = (mode & OT_RUN_TIME_LAUNCHES) != 0;
+  const bool async = (mode & OT_RUN_ASYNC) != 0;
   if (use_graph && time_launches)
     return set_error(OT_EINVAL, "OT_RUN_TIME_LAUNCHES times eager launches; do not combine with OT_RUN_GRAPH");
+  if (async && time_launches)
+    return set_error(OT_EINVAL, "OT_RUN_TIME_LAUNCHES waits for the launches; do not combine with OT_RUN_ASYNC");
   if (time_launches && pl->nlev < 2 * pl->d.launches) {
     for (int i = 0; i < pl->nlev; ++i) (void)hipEventDestroy(pl->lev[i]);
     delete[] pl->lev;
@@ -347,6 +357,18 @@ ot_status ot_run(ot_plan* pl, float* const* rw_a, float* const* rw_b, const floa
       pl->graph_stream = s;
       std::memcpy(pl->graph_key, key, sizeof(key));
     }
+  }
+  if (async) {
+    // enqueue only: no events, no wait (the caller synchronises the stream)
+    if (use_graph) {
+      OT_HIP(hipGraphLaunch(pl->exec, s));
+    } else {
+      st = enqueue_loop(pl, rw_a, rw_b, ro, s, nullptr);
+      if (st != OT_OK) return st;
+    }
+    tm.full_launches = pl->p.time_steps / pl->p.time_tile;
+    if (timing) *timing = tm;
+    return OT_OK;
   }
   const auto t0 = std::chrono::steady_clock::now();
   OT_HIP(hipEventRecord(pl->ev0, s));

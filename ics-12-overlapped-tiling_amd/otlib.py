@@ -71,7 +71,7 @@ class Timing(C.Structure):
                 ("reserved", C.c_int32)]
 
 
-RUN_GRAPH, RUN_TIME_LAUNCHES = 1, 2
+RUN_GRAPH, RUN_TIME_LAUNCHES, RUN_ASYNC = 1, 2, 8
 
 
 class Image(C.Structure):
@@ -109,6 +109,7 @@ SIGNATURES = [
     ("ot_plan_create", C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
     ("ot_plan_derived", C.c_int, [C.c_void_p, C.POINTER(Derived)]),
     ("ot_plan_destroy", None, [C.c_void_p]),
+    ("ot_plan_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ot_run", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_void_p, C.c_int32, C.POINTER(Timing)]),
     ("ot_launch", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_int32, C.c_void_p]),
     ("ot_naive_run", C.c_int, [C.POINTER(Params), _PP, _PP, _PP, C.c_void_p, C.POINTER(Timing)]),
@@ -326,13 +327,16 @@ class Plan:
         self.derived = Derived()
         check(lib().ot_plan_derived(h, C.byref(self.derived)))
 
-    def run(self, rw_a, rw_b, ro=(), use_graph=False, time_launches=False, stream=None) -> Timing:
-        """rw_a/rw_b/ro: sequences of DeviceBuffer. Returns the Timing."""
+    def run(self, rw_a, rw_b, ro=(), use_graph=False, time_launches=False, stream=None,
+            enqueue_only=False) -> Timing:
+        """rw_a/rw_b/ro: sequences of DeviceBuffer. Returns the Timing.
+        enqueue_only: OT_RUN_ASYNC (no wait, no timing; sync with ot_dev_sync)."""
         a = _ptr_array([b.ptr for b in rw_a])
         b = _ptr_array([x.ptr for x in rw_b])
         r = _ptr_array([x.ptr for x in ro]) if ro else None
         tm = Timing()
-        mode = (RUN_GRAPH if use_graph else 0) | (RUN_TIME_LAUNCHES if time_launches else 0)
+        mode = ((RUN_GRAPH if use_graph else 0) | (RUN_TIME_LAUNCHES if time_launches else 0) |
+                (RUN_ASYNC if enqueue_only else 0))
         check(lib().ot_run(self.handle, a, b, r, stream, mode, C.byref(tm)))
         return tm
 

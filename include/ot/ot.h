@@ -159,8 +159,11 @@ typedef struct ot_timing {
 enum {
   OT_RUN_GRAPH = 1,         /* capture the loop into a hipGraph once, replay it */
   OT_RUN_TIME_LAUNCHES = 2, /* bracket every launch with HIP events (eager only) */
-  OT_RUN_PER_IMAGE = 4      /* ot_batch_run: one per-grid launch per image and time
+  OT_RUN_PER_IMAGE = 4,     /* ot_batch_run: one per-grid launch per image and time
                                tile instead of the work-queue kernel (baseline) */
+  OT_RUN_ASYNC = 8          /* ot_run: enqueue the loop (or the graph launch) and
+                               return without waiting or timing; the caller
+                               synchronises the stream (back-to-back solves) */
 };
 
 typedef struct ot_plan ot_plan;
@@ -186,6 +189,9 @@ ot_status ot_derive(const ot_params* params, ot_derived* derived);
 ot_status ot_plan_create(const ot_params* params, ot_plan** plan);
 ot_status ot_plan_derived(const ot_plan* plan, ot_derived* derived);
 void ot_plan_destroy(ot_plan* plan);
+/* The plan's own non-blocking stream (created on first use), on which
+ * ot_run(..., stream = NULL, OT_RUN_GRAPH) replays its graph. */
+ot_status ot_plan_stream(ot_plan* plan, void** stream);
 
 /* Run the whole time loop (replaces opencl/jacobi-2d/jacobi-2d.cpp:903-933):
  * ceil(steps/T) launches ping-ponging between the rw_a and rw_b field sets,
@@ -195,7 +201,9 @@ void ot_plan_destroy(ot_plan* plan);
  * loop once into a hipGraph and replays it on later calls with the same
  * buffers and stream; OT_RUN_TIME_LAUNCHES times every launch with its own
  * event pair. On return the stream is synchronised and timing (optional) is
- * filled. */
+ * filled — except with OT_RUN_ASYNC, which only enqueues (timing then holds
+ * the launch counts and final_buffer). With OT_RUN_GRAPH and a NULL stream the
+ * graph runs on a stream the plan owns; ot_plan_stream returns it. */
 ot_status ot_run(ot_plan* plan, float* const* rw_a, float* const* rw_b,
                  const float* const* ro, void* stream, int32_t mode,
                  ot_timing* timing);

@@ -174,3 +174,26 @@ def test_reference_geometry_layout(otlib, st):
     ref = oracle.run(st, [init[otlib.interior_slices(d)].copy()], steps=6)[0]
     m = oracle.compare(ref, out[otlib.interior_slices(d)])
     assert m["max_rel"] <= MAX_REL and m["bitwise_equal"], m
+
+
+def test_enqueue_only_solves_match_synchronous(otlib):
+    """OT_RUN_ASYNC enqueues the same loop (eager or graph replay) without
+    waiting; back-to-back solves from the same input give the synchronous
+    result."""
+    p = _params(otlib.J3D, (70, 37, 35), 7, 2, (60, 12, 8), 1)
+    d = otlib.derive(p)
+    rw, _ = random_inputs(otlib.J3D, domain_shape(d), seed=9)
+    ref, _ = run_gpu(p, rw)
+    plan = otlib.Plan(p)
+    full = otlib.empty_field(d)
+    full[otlib.interior_slices(d)] = rw[0]
+    for use_graph in (False, True):
+        A = otlib.DeviceBuffer.from_array(full)
+        B = otlib.DeviceBuffer.from_array(full)
+        tm = plan.run([A], [B], use_graph=use_graph, enqueue_only=True)
+        otlib.check(otlib.lib().ot_dev_sync())
+        out = (B if tm.final_buffer else A).download(full.shape)[otlib.interior_slices(d)]
+        assert np.array_equal(out.view(np.uint32), ref[0].view(np.uint32)), use_graph
+        A.free()
+        B.free()
+    plan.close()
