@@ -197,3 +197,32 @@ def test_enqueue_only_solves_match_synchronous(otlib):
         A.free()
         B.free()
     plan.close()
+
+
+FULL_SIZE = [
+    # (stencil, extents, steps, tiled (T, block, ept), T=1 (block, ept)): the bench workloads
+    (otlib.J3D, (256, 256, 256), 32, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
+    (otlib.G3D, (256, 256, 256), 16, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
+    (otlib.P3D7, (256, 256, 256), 32, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
+    (otlib.FDTD2D, (2048, 2048, 1), 100, (4, (64, 8, 1), 2), ((64, 4, 1), 4)),
+    (otlib.J2D, (1024, 1024, 1), 64, (4, (16, 16, 1), 1), ((64, 4, 1), 4)),
+]
+
+
+@pytest.mark.parametrize("case", FULL_SIZE, ids=[f"{otlib.STENCILS[c[0]]}-full" for c in FULL_SIZE])
+def test_full_size_time_tiling_is_bitwise_t1(otlib, case):
+    """At the measured workload sizes the time-tiled solve equals the T=1 solve
+    bit for bit (size-independent property: same operations in the same order),
+    and the T=1 solve's first step equals the oracle's."""
+    st, n3, steps, (T, blk, ept), (blk1, ept1) = case
+    p = _params(st, n3, steps, T, blk, ept)
+    p1 = _params(st, n3, steps, 1, blk1, ept1)
+    d = otlib.derive(p)
+    rw, ro = random_inputs(st, domain_shape(d), seed=31)
+    print(f"full-size {otlib.STENCILS[st]} {n3} steps={steps} T={T} {blk}", flush=True)
+    tiled, _ = run_gpu(p, rw, ro)
+    base, _ = run_gpu(p1, rw, ro)
+    for a, b in zip(tiled, base):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    one = _params(st, n3, 1, T, blk, ept)
+    check_against_oracle(one, rw, ro)
