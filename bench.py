@@ -57,8 +57,12 @@ WORKLOADS = {
                           tiled=dict(T=2, x=64, y=8, z=32, e=2), t1=dict(T=1, x=32, y=8, z=8, e=2),
                           bytes_pp=12, data="u0 = 0, f uniform [-1, 1) from seed 11"),
 }
-# batch workloads: (T, block x, block y rows, elements per thread)
-DEFAULT_BATCH = {"batch-rician": (3, 32, 8, 4), "batch-tv": (2, 64, 4, 4)}
+# batch workloads: (T, block x, block y rows, elements per thread) of the
+# batched work-queue launch, and of the per-image baseline (one launch per image
+# per time tile) at the configuration that is fastest for per-image launches
+# (both from sweeps of the batch on an MI355X)
+DEFAULT_BATCH = {"batch-rician": (2, 64, 8, 4), "batch-tv": (2, 48, 8, 4)}
+PER_IMAGE_BEST = {"batch-rician": (3, 32, 8, 4), "batch-tv": (4, 32, 16, 2)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jacobi3d_256_traffic.json")
 MODEL_FILE = os.path.join(ROOT, "ics-12-overlapped-tiling_amd", "tuning", "mi355x", "mlp.bin")
 
@@ -193,7 +197,9 @@ def cpu_baseline(w, n3, steps):
 def bench_batch(args, stencil_name):
     """Batched Rician / TV denoising of uneven images: one persistent
     work-queue launch per time tile (hipGraph replay) against the same images
-    advanced by one per-grid launch per image and time tile."""
+    advanced by one per-grid launch per image and time tile, both with the
+    batch's tile configuration and with the configuration fastest for
+    per-image launches."""
     import otbatch
 
     st = otlib.RICIAN2D if stencil_name == "rician" else otlib.TV2D
@@ -203,25 +209,29 @@ def bench_batch(args, stencil_name):
     coef = [0.05, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
     T, bx, by, e = (args.T, args.bx, args.by, args.ept)
     p = otlib.make_params(st, steps=args.iterations, T=T, x=bx, y=by, e=e, coef=coef)
+    Tq, bxq, byq, eq = PER_IMAGE_BEST["batch-" + stencil_name]
+    q = otlib.make_params(st, steps=args.iterations, T=Tq, x=bxq, y=byq, e=eq, coef=coef)
     ro_host = [f] if st == otlib.RICIAN2D else [np.ones(total, np.float32), f]
     A = otlib.DeviceBuffer.from_array(f)
     B = otlib.DeviceBuffer.from_array(f)
     RO = [otlib.DeviceBuffer.from_array(x) for x in ro_host]
     bt = otbatch.Batch(p, ims)
+    bq = otbatch.Batch(q, ims)
 
-    def timed(use_graph, per_image, reps):
+    def timed(b, use_graph, per_image, reps):
         for _ in range(args.warmup):
-            bt.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
+            b.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
         t0 = time.perf_counter()
         for _ in range(reps):
-            bt.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
+            b.run([A], [B], RO, use_graph=use_graph, per_image=per_image)
         otlib.check(otlib.lib().ot_dev_sync())
         return (time.perf_counter() - t0) / reps
 
-    batched = timed(True, False, args.steps)
-    batched_eager = timed(False, False, args.steps)
-    per_image = timed(False, True, max(2, args.steps // 2))
-    per_image_graph = timed(True, True, max(2, args.steps // 2))
+    batched = timed(bt, True, False, args.steps)
+    batched_eager = timed(bt, False, False, args.steps)
+    per_image = timed(bt, False, True, max(2, args.steps // 2))
+    per_image_graph = timed(bt, True, True, max(2, args.steps // 2))
+    per_image_best = timed(bq, False, True, max(2, args.steps // 2))
     pixels = float(sum(int(im.width) * int(im.height) for im in ims))
     out = {
         "metric": f"batched {stencil_name} denoise Gpixel-iterations/s",
@@ -240,8 +250,12 @@ def bench_batch(args, stencil_name):
         "per_image_launches": int(len(ims) * bt.launches),
         "speedup_vs_per_image": round(per_image / batched, 3),
         "speedup_vs_per_image_graph": round(per_image_graph / batched, 3),
+        "per_image_best_config": {"time_tile": Tq, "tile": [bxq, byq * eq], "elements_per_thread": eq,
+                                  "ms": round(per_image_best * 1e3, 4)},
+        "speedup_vs_per_image_best": round(per_image_best / batched, 3),
     }
     bt.close()
+    bq.close()
     print(json.dumps(out), flush=True)
 
 

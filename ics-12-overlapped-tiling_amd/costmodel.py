@@ -3,6 +3,7 @@
     python costmodel.py collect  --out DIR [--per-stencil K] [--seed S]
     python costmodel.py train    --data DIR --model FILE
     python costmodel.py evaluate --data DIR --model FILE --report FILE
+    python costmodel.py loso     --data DIR --report FILE
 
 collect  runs the exhaustive autotuner (ot_tune_exhaustive) over seeded random
          problem shapes of every stencil (1D up to 2^24 points, 2D up to 4096^2,
@@ -17,11 +18,15 @@ evaluate ranks every held-out shape's configuration space with the model and
          measured best (the metric of run-model.py:575-623 and
          scripts/parse-weka-results.py:28-35) and the tuning time of
          "predict, then measure the top k" against the exhaustive sweep.
+loso     the reference's leave-one-stencil-out protocol: one model per
+         stencil, trained without it, ranking all of its shapes.
 
 The reference's learned precedent is WEKA IBk over 17 features with a
 leave-one-stencil-out protocol (scripts/extract-features.py:40-209,
-scripts/run-weka.py:74-149); this model is a 24-feature MLP and its held-out
-set is unseen shapes of every stencil.
+scripts/run-weka.py:74-149); this model is a 28-feature MLP (one-hot stencil,
+shape, tile, stencil descriptors and the analytic model's estimate as a prior)
+and its held-out set is unseen shapes of every stencil; `loso` repeats the
+reference's protocol with the one-hot block zeroed.
 """
 import argparse
 import json
@@ -115,13 +120,19 @@ def split(groups):
     return train, test
 
 
-def matrix(groups, keys):
+ONEHOT = 13  # leading one-hot stencil columns of the feature vector
+
+
+def matrix(groups, keys, mask_onehot=False):
     X, y = [], []
     for k in keys:
         for r in groups[k]:
             X.append(ottune.features(ottune.params_of_row(r)))
             y.append(ottune.label_of_row(r))
-    return np.asarray(X, np.float32), np.asarray(y, np.float32)
+    X = np.asarray(X, np.float32)
+    if mask_onehot:
+        X[:, :ONEHOT] = 0.0
+    return X, np.asarray(y, np.float32)
 
 
 def train(data_dir: str, model_path: str, epochs: int, batch: int, lr: float, seed: int):
@@ -149,55 +160,98 @@ def solve_cost(r, warmup=1, reps=3):
     return (warmup + reps) * float(r["EventElapsed"])
 
 
-def evaluate(data_dir: str, model_path: str, report_path: str, top_k=(1, 3)):
-    groups = load(data_dir)
-    sweeps_path = os.path.join(data_dir, "sweeps.json")
-    sweeps = json.load(open(sweeps_path)) if os.path.exists(sweeps_path) else {}
-    _, te = split(groups)
-    m = ottune.CostModel.load(model_path)
-    arch = ottune.default_arch()
-    per_shape = []
-    for k in te:
-        rows = groups[k]
-        params = [ottune.params_of_row(r) for r in rows]
-        t = np.array([float(r["EventElapsed"]) for r in rows])
-        best = float(t.min())
-        t0 = time.perf_counter()
-        X = np.stack([ottune.features(p) for p in params])
-        pred = m.predict(X)
-        infer_s = time.perf_counter() - t0
-        order = np.argsort(pred, kind="stable")
-        ana = np.array([ottune.analytic_seconds(p, int(r["Register Usage"]), arch) for p, r in zip(params, rows)])
-        aorder = np.argsort(ana, kind="stable")
-        n_top1pct = max(1, int(math.ceil(0.01 * len(rows))))
-        exhaustive_s = float(sum(solve_cost(r) for r in rows))
-        rec = {"shape": k, "configs": len(rows), "best_s": best,
-               "exhaustive_solve_s": exhaustive_s, "exhaustive_wall_s": sweeps.get(k, {}).get("wall_s"),
-               "inference_s": infer_s,
-               "analytic_top1_slowdown": float(t[aorder[0]] / best),
-               "analytic_top1pct_slowdown": float(t[aorder[:n_top1pct]].min() / best),
-               "best_config": {n: rows[int(t.argmin())][n] for n in
-                               ("Time Tile Size", "Block Size X", "Block Size Y", "Block Size Z", "Elements/Thread")}}
-        for kk in top_k:
-            sel = order[:kk]
-            rec[f"learned_top{kk}_slowdown"] = float(t[sel].min() / best)
-            learned_cost = infer_s + float(sum(solve_cost(rows[i]) for i in sel))
-            rec[f"learned_top{kk}_tuning_ratio"] = learned_cost / exhaustive_s
-            if rec["exhaustive_wall_s"]:
-                rec[f"learned_top{kk}_wall_ratio"] = learned_cost / rec["exhaustive_wall_s"]
-        per_shape.append(rec)
+def rank_shape(m, key, rows, arch, sweeps, top_k=(1, 3), mask_onehot=False):
+    """Learned and analytic picks for one shape's measured rows."""
+    params = [ottune.params_of_row(r) for r in rows]
+    t = np.array([float(r["EventElapsed"]) for r in rows])
+    best = float(t.min())
+    t0 = time.perf_counter()
+    X = np.stack([ottune.features(p) for p in params])
+    if mask_onehot:
+        X[:, :ONEHOT] = 0.0
+    pred = m.predict(X)
+    infer_s = time.perf_counter() - t0
+    order = np.argsort(pred, kind="stable")
+    ana = np.array([ottune.analytic_seconds(p, int(r["Register Usage"]), arch) for p, r in zip(params, rows)])
+    aorder = np.argsort(ana, kind="stable")
+    n_top1pct = max(1, int(math.ceil(0.01 * len(rows))))
+    exhaustive_s = float(sum(solve_cost(r) for r in rows))
+    rec = {"shape": key, "configs": len(rows), "best_s": best,
+           "exhaustive_solve_s": exhaustive_s, "exhaustive_wall_s": sweeps.get(key, {}).get("wall_s"),
+           "inference_s": infer_s,
+           "analytic_top1_slowdown": float(t[aorder[0]] / best),
+           "analytic_top1pct_slowdown": float(t[aorder[:n_top1pct]].min() / best),
+           "best_config": {n: rows[int(t.argmin())][n] for n in
+                           ("Time Tile Size", "Block Size X", "Block Size Y", "Block Size Z", "Elements/Thread")}}
+    for kk in top_k:
+        sel = order[:kk]
+        rec[f"learned_top{kk}_slowdown"] = float(t[sel].min() / best)
+        learned_cost = infer_s + float(sum(solve_cost(rows[i]) for i in sel))
+        rec[f"learned_top{kk}_tuning_ratio"] = learned_cost / exhaustive_s
+        if rec["exhaustive_wall_s"]:
+            rec[f"learned_top{kk}_wall_ratio"] = learned_cost / rec["exhaustive_wall_s"]
+    return rec
 
+
+def summarize(per_shape, top_k=(1, 3)):
     def agg(field):
         v = np.array([r[field] for r in per_shape if r.get(field) is not None], float)
         return {"mean": float(v.mean()), "median": float(np.median(v)), "max": float(v.max()),
                 "within_10pct": float(np.mean(v <= 1.10))} if len(v) else None
 
-    summary = {f: agg(f) for f in ["analytic_top1_slowdown", "analytic_top1pct_slowdown"] +
-               [f"learned_top{k}_slowdown" for k in top_k] + [f"learned_top{k}_tuning_ratio" for k in top_k] +
-               [f"learned_top{k}_wall_ratio" for k in top_k]}
+    return {f: agg(f) for f in ["analytic_top1_slowdown", "analytic_top1pct_slowdown"] +
+            [f"learned_top{k}_slowdown" for k in top_k] + [f"learned_top{k}_tuning_ratio" for k in top_k] +
+            [f"learned_top{k}_wall_ratio" for k in top_k]}
+
+
+def _sweeps(data_dir):
+    path = os.path.join(data_dir, "sweeps.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+def evaluate(data_dir: str, model_path: str, report_path: str):
+    groups = load(data_dir)
+    sweeps = _sweeps(data_dir)
+    _, te = split(groups)
+    m = ottune.CostModel.load(model_path)
+    arch = ottune.default_arch()
+    per_shape = [rank_shape(m, k, groups[k], arch, sweeps) for k in te]
+    summary = summarize(per_shape)
     report = {"held_out_shapes": len(per_shape), "summary": summary, "per_shape": per_shape}
     json.dump(report, open(report_path, "w"), indent=1)
     print(json.dumps(summary), flush=True)
+    return report
+
+
+def loso(data_dir: str, report_path: str, epochs: int, batch: int, lr: float, seed: int):
+    """Leave-one-stencil-out (the reference's protocol, scripts/run-weka.py:74-149):
+    for every stencil, train on all shapes of the other stencils and rank every
+    shape of the left-out one."""
+    groups = load(data_dir)
+    sweeps = _sweeps(data_dir)
+    arch = ottune.default_arch()
+    stencils = sorted({k.split(":")[0] for k in groups})
+    per_stencil, all_shapes = {}, []
+    for st in stencils:
+        tr = [k for k in groups if k.split(":")[0] != st]
+        te = sorted(k for k in groups if k.split(":")[0] == st)
+        X, y = matrix(groups, tr, mask_onehot=True)
+        m = ottune.CostModel(seed=seed)
+        m.fit_normalization(X)
+        m.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        recs = [rank_shape(m, k, groups[k], arch, sweeps, mask_onehot=True) for k in te]
+        m.close()
+        per_stencil[st] = summarize(recs)
+        all_shapes += recs
+        s1 = per_stencil[st]["learned_top1_slowdown"]
+        s3 = per_stencil[st]["learned_top3_slowdown"]
+        sa = per_stencil[st]["analytic_top1_slowdown"]
+        print(f"{st}: learned top-1 mean {s1['mean']:.3f}, top-3 mean {s3['mean']:.3f}, "
+              f"analytic top-1 mean {sa['mean']:.3f} over {len(recs)} shapes", flush=True)
+    report = {"protocol": "leave-one-stencil-out", "epochs": epochs, "summary": summarize(all_shapes),
+              "per_stencil": per_stencil, "per_shape": all_shapes}
+    json.dump(report, open(report_path, "w"), indent=1)
+    print(json.dumps(report["summary"]), flush=True)
     return report
 
 
@@ -220,13 +274,22 @@ def main():
     e.add_argument("--data", required=True)
     e.add_argument("--model", required=True)
     e.add_argument("--report", required=True)
+    lo = sub.add_parser("loso")
+    lo.add_argument("--data", required=True)
+    lo.add_argument("--report", required=True)
+    lo.add_argument("--epochs", type=int, default=150)
+    lo.add_argument("--batch", type=int, default=256)
+    lo.add_argument("--lr", type=float, default=1e-3)
+    lo.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
     if a.cmd == "collect":
         collect(a.out, a.per_stencil, a.seed, a.time_limit)
     elif a.cmd == "train":
         train(a.data, a.model, a.epochs, a.batch, a.lr, a.seed)
-    else:
+    elif a.cmd == "evaluate":
         evaluate(a.data, a.model, a.report)
+    else:
+        loso(a.data, a.report, a.epochs, a.batch, a.lr, a.seed)
 
 
 if __name__ == "__main__":

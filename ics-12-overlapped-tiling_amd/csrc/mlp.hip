@@ -1,11 +1,11 @@
-// mlp.hip — the learned tile-configuration cost model: a 24 -> 64 -> 64 -> 64
+// mlp.hip — the learned tile-configuration cost model: a 28 -> 64 -> 64 -> 64
 // -> 1 ReLU perceptron predicting log(seconds per grid point per time step),
 // with forward, backward, Adam and inference kernels.
 //
 // No reference counterpart in kernel form: the reference's learned ranking is a
 // WEKA IBk regressor run in a JVM (scripts/run-weka.py:74-149) over 17
 // features of the driver output (scripts/extract-features.py:40-209). The
-// model is small (9985 parameters), so every kernel is a plain one-thread-per-
+// model is small (10241 parameters), so every kernel is a plain one-thread-per-
 // output-element loop; reductions run in a fixed order (no atomics), which
 // makes training and inference bit-reproducible for a given seed.
 #include <hip/hip_runtime.h>

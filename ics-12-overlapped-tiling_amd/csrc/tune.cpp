@@ -137,6 +137,18 @@ ot_status ot_features(const ot_params* p, float* f) {
   f[21] = (float)log2d(p->block_z);
   f[22] = (float)log2d(p->elems_per_thread);
   f[23] = (d.dims == 3 && ot::k3w_rows(p->time_tile, d.tile) > 0) ? 1.0f : 0.0f;
+  // stencil descriptors, so that a stencil absent from the training data is
+  // still placed among the others, and the analytic model as a prior
+  f[24] = (float)d.dims;
+  f[25] = (float)tr.flops;
+  f[26] = (float)tr.f_p2_gl;
+  ot_arch arch;
+  ot_arch_default(&arch);
+  double est = 0.0;
+  if (ot_model_analytic(p, &arch, 0, &est) == OT_OK && est > 0.0) {
+    const double pts = (double)d.real_size[0] * d.real_size[1] * d.real_size[2];
+    f[27] = (float)std::log(est / (pts * std::max(1, p->time_steps)));
+  }
   return OT_OK;
 }
 

@@ -95,7 +95,7 @@ class Arch(C.Structure):
 
 _FP = C.POINTER(C.c_float)
 _PP = C.POINTER(_FP)
-FEATURES = 24
+FEATURES = 28
 
 # (name, restype, argtypes) of every entry point declared in include/ot/*.h
 SIGNATURES = [

@@ -15,7 +15,7 @@
  *   ot_tune_exhaustive  — run_experiment: every configuration, a resumable CSV
  *                         with the driver's stdout keys as columns
  *   ot_model_analytic   — run-model.py's per-configuration estimate
- *   ot_model_*          — the learned replacement: 24 features -> 3 x 64 ReLU
+ *   ot_model_*          — the learned replacement: 28 features -> 3 x 64 ReLU
  *                         -> log(seconds per point-step), trained and evaluated
  *                         by HIP kernels.
  */
@@ -93,8 +93,11 @@ ot_status ot_model_analytic(const ot_params* p, const ot_arch* arch, int32_t reg
 
 /* ---- learned cost model ------------------------------------------------------- */
 
-#define OT_FEATURES 24   /* one-hot stencil (13), log2 nx/ny/nz, radius, fields,
-                            T, log2 block x / rows / z, log2 ept, wave kernel */
+#define OT_FEATURES 28   /* one-hot stencil (13), log2 nx/ny/nz, radius, fields,
+                            T, log2 block x / rows / z, log2 ept, wave kernel,
+                            dims, flops per point, loads per point, and the
+                            natural log of ot_model_analytic's seconds per
+                            point-step (registers unknown: 0) */
 #define OT_MLP_HIDDEN 64 /* three hidden ReLU layers */
 
 /* Feature vector of one configuration (the input of ot_predict). */
@@ -107,8 +110,8 @@ typedef struct ot_model ot_model;
 ot_status ot_model_create(uint64_t seed, ot_model** model);
 void ot_model_destroy(ot_model* model);
 
-/* Flat parameter vector: W1[24x64] b1[64] W2[64x64] b2[64] W3[64x64] b3[64]
- * W4[64] b4[1] (row-major, input index first) — 9985 floats. */
+/* Flat parameter vector: W1[28x64] b1[64] W2[64x64] b2[64] W3[64x64] b3[64]
+ * W4[64] b4[1] (row-major, input index first) — 10241 floats. */
 ot_status ot_model_param_count(const ot_model* model, int32_t* count);
 ot_status ot_model_get_params(const ot_model* model, float* host, int32_t count);
 ot_status ot_model_set_params(ot_model* model, const float* host, int32_t count);

@@ -1,8 +1,8 @@
 """FP64 CPU restatement of the learned cost model (test infrastructure only).
 
 Checker for the HIP kernels in ics-12-overlapped-tiling_amd/csrc/mlp.hip: the
-same 24 -> 64 -> 64 -> 64 -> 1 ReLU perceptron over the same flat parameter
-layout (W1[24x64] b1 W2[64x64] b2 W3[64x64] b3 W4[64] b4, row-major, input
+same 28 -> 64 -> 64 -> 64 -> 1 ReLU perceptron over the same flat parameter
+layout (W1[28x64] b1 W2[64x64] b2 W3[64x64] b3 W4[64] b4, row-major, input
 index first), the same feature normalisation x' = (x - mean) * inv_std, the
 mean-squared-error loss, its gradient by backpropagation, and one Adam step
 (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected). Everything is float64.
@@ -14,7 +14,7 @@ tests/test_mlp_oracle.py.
 """
 import numpy as np
 
-F = 24
+F = 28
 H = 64
 SHAPES = [("W1", (F, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, H)), ("b3", (H,)),
           ("W4", (H,)), ("b4", (1,))]

@@ -16,7 +16,7 @@ def _random_params(rng, scale=0.3):
 
 
 def test_parameter_count_matches_layout():
-    assert M.NP == 24 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1 == 9985
+    assert M.NP == 28 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1 == 10241
 
 
 def test_gradient_matches_central_differences():

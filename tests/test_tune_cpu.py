@@ -38,13 +38,17 @@ def test_space_holds_both_3d_kernel_families(otlib):
 
 
 def test_features(otlib):
-    f = ottune.features(_p("jacobi-3d", (256, 128, 64), 32, 3, 58, 6, 20, 1))
-    assert f.shape == (24,)
+    p = _p("jacobi-3d", (256, 128, 64), 32, 3, 58, 6, 20, 1)
+    f = ottune.features(p)
+    assert f.shape == (28,)
     assert f[4] == 1.0 and f[:13].sum() == 1.0
     assert tuple(f[13:16]) == (8.0, 7.0, 6.0)
     assert f[16] == 1 and f[17] == 1 and f[18] == 3
     assert f[19] == pytest.approx(math.log2(58)) and f[20] == pytest.approx(math.log2(6))
     assert f[21] == pytest.approx(math.log2(20)) and f[22] == 0.0 and f[23] == 1.0
+    assert tuple(f[24:27]) == (3.0, 7.0, 7.0)  # dims, flops per point, loads per point
+    prior = ottune.analytic_seconds(p, 0) / (256 * 128 * 64 * 32)
+    assert f[27] == pytest.approx(math.log(prior), rel=1e-6)
     g = ottune.features(_p("poisson-3d-rhs", (64, 64, 64), 8, 2, 32, 4, 16, 4))
     assert g[7] == 1.0 and g[17] == 2 and g[22] == 2.0 and g[23] == 0.0
 
