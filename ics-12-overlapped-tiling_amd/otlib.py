@@ -131,6 +131,7 @@ SIGNATURES = [
     ("ot_fill_uniform", C.c_int, [_FP, C.POINTER(Derived), C.c_uint64, C.c_float, C.c_float]),
     ("ot_fill_sinusoid_noise", C.c_int, [_FP, C.POINTER(Derived), C.c_uint64, C.c_float]),
     ("ot_fill_gaussian_pulse", C.c_int, [_FP, C.POINTER(Derived), C.c_float, C.c_float]),
+    ("ot_selftest_rsq", C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     # ot_tune.h
     ("ot_tune_space", C.c_int, [C.POINTER(Params), C.POINTER(Params), C.c_int32, C.POINTER(C.c_int32)]),
     ("ot_tune_measure", C.c_int, [C.POINTER(Params), _PP, _PP, _PP, C.c_int32, C.c_int32, C.c_double,

@@ -279,6 +279,15 @@ ot_status ot_fill_uniform(float* host, const ot_derived* d, uint64_t seed, float
 ot_status ot_fill_sinusoid_noise(float* host, const ot_derived* d, uint64_t seed, float noise_amp);
 ot_status ot_fill_gaussian_pulse(float* host, const ot_derived* d, float amplitude, float sigma);
 
+/* ---- self-test ------------------------------------------------------------
+ * No reference counterpart. The kernels evaluate 1.0f/sqrtf(x) (the reference
+ * CPU loop's rsqrt, gradient-2d.cpp:697-699) from hardware estimates plus
+ * corrections; this entry point evaluates that sequence and the IEEE
+ * expression on every float bit pattern in [lo_bits, hi_bits] on the device
+ * and returns how many differ bit for bit (NaN == NaN) and the smallest
+ * differing pattern (0xFFFFFFFF if none). */
+ot_status ot_selftest_rsq(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches, uint32_t* first_bad);
+
 #ifdef __cplusplus
 }
 #endif
