@@ -52,6 +52,18 @@ __device__ __forceinline__ float rsq(float x) {
 }
 __device__ __forceinline__ float sq(float x) { return x * x; }
 
+// Wavefront x-neighbours for the wave-per-tile kernels (lanes = consecutive x
+// columns): DPP wave_shr:1 / wave_shl:1 with bound_ctrl, folded by the compiler
+// into the consuming VALU instruction.
+// lane i receives lane i-1's value (lane 0 receives 0)
+__device__ __forceinline__ float from_left(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+// lane i receives lane i+1's value (lane 63 receives 0)
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
 // jacobi-1d.cpp:633-635
 __device__ __forceinline__ float j1d3(float l, float c, float r) { return 0.333f * ((l + c) + r); }
 // jacobi-1d-5pt.cpp:507-511

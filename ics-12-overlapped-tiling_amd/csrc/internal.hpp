@@ -28,7 +28,7 @@ struct LaunchGeom {
   int32_t phase_limit;
   int32_t variant;
   float coef[4];
-  int32_t wave_rows;     // 3D: footprint rows of the wave-pipelined kernel, 0 = LDS kernel
+  int32_t wave_rows;     // footprint rows of the wave-per-tile kernel (2D, 3D), 0 = LDS kernel
   int32_t reserved;
 };
 
@@ -38,6 +38,14 @@ constexpr int kW3RegBudget = 192;
 // Footprint rows of the wave-pipelined 3D kernel for this tile, or 0 when the
 // tile does not fill a wavefront (x tile + 2T == 64) or has no compiled row count.
 int k3w_rows(int T, const int32_t tile[3]);
+
+// Register budget (fields x rows per lane) of the wave-per-tile 2D kernels.
+constexpr int kW2RegBudget = 128;
+
+// Footprint rows of the wave-per-tile 2D kernel for this stencil and tile, or 0
+// when the tile does not fill a wavefront (x tile + 2*radius*T == 64) or has no
+// compiled row count.
+int k2w_rows(int stencil, int T, const int32_t tile[3]);
 
 // Per-launch field pointers (up to 3 rw fields, 2 ro fields).
 struct FieldPtrs {
@@ -64,6 +72,12 @@ ot_status launch_3d_g3d(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs
 ot_status launch_3d_p3d7(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3d_p3d27(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_j3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_j2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_p2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_g2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_tv2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_fdtd2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2dw_rician2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_g3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_p3d7(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_p3d27(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);

@@ -1,0 +1,8 @@
+// k2w_g2d.hip — wave-per-tile 2D kernel instantiations for OT_G2D (k2w_impl.hpp).
+#include "k2w_impl.hpp"
+
+namespace ot {
+ot_status launch_2dw_g2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  return k2w::launch_st<OT_G2D>(t_eff, rows, g, f, stream, prepare);
+}
+}  // namespace ot

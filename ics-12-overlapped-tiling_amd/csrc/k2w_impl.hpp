@@ -1,0 +1,224 @@
+// k2w_impl.hpp — wave-per-tile overlapped time-tiled kernels for the 2D stencils.
+//
+// A second kernel family for the 2D stencils (jacobi-2d.cpp:175-514,
+// poisson-2d.cpp:127-364, gradient-2d.cpp:114-393, fdtd-2d.cpp:107-443,
+// rician-denoise-2d.cpp:108-447, cdsc-tv-update-2d.cpp:116-412) with no LDS
+// and no barriers: every wavefront is an independent tile.
+//   * The 64 lanes are 64 consecutive x columns of the footprint; every lane
+//     holds its whole column strip of FR footprint rows of every field in
+//     registers. x-neighbours come from the adjacent lanes through DPP wave
+//     shifts, y-neighbours are the lane's own neighbouring rows.
+//   * All t_eff steps are advanced in registers. Step s recomputes only the
+//     rows still needed by later steps (the valid region shrinks by the
+//     stencil's radius per step), so the redundant work is the x-halo of 2H
+//     lanes plus a shrinking y-halo (H = radius * t_eff).
+//   * Output tile = (64 - 2H) x (FR - 2H) points; every point outside the
+//     domain is forced to +0 after every step (Dirichlet 0).
+// Multi-stage stencils keep the reference's per-step order: FDTD updates Ey, Ex
+// from Hz, then Hz from the new Ex, Ey (fdtd-2d.cpp:269-330); Rician computes g
+// from u, then u from g and u (rician-denoise-2d.cpp:271-327).
+#pragma once
+#include "internal.hpp"
+#include "device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ot {
+namespace k2w {
+
+using namespace dev;
+
+template <int ST>
+struct KW2 {
+  static constexpr int R = (ST == OT_RICIAN2D) ? 2 : 1;
+  static constexpr int NRW = (ST == OT_FDTD2D) ? 3 : 1;
+  static constexpr int NRO = (ST == OT_TV2D) ? 2 : ((ST == OT_RICIAN2D) ? 1 : 0);
+};
+
+constexpr int kWaves = 4;
+
+template <int ST, int T, int FR>
+__global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
+  constexpr int R = KW2<ST>::R;
+  constexpr int NRW = KW2<ST>::NRW;
+  constexpr int NRO = KW2<ST>::NRO;
+  constexpr int H = R * T;
+  static_assert(FR > 2 * H, "footprint rows must exceed the halo");
+  const int lane = threadIdx.x & 63;
+  // wave-uniform by construction; readfirstlane keeps the tile origin in SGPRs
+  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, (int)gridDim.x) * kWaves +
+                                                 (int)(threadIdx.x >> 6));
+  const int nw = g.grid[0] * g.grid[1];
+  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
+  const int bxi = wid % g.grid[0], byi = wid / g.grid[0];
+  // tile (TX, TY) with TX + 2H <= 64 lanes and TY + 2H <= FR rows; a plan uses
+  // TX = 64 - 2H and FR = TY + 2H, a shortened last launch (smaller t_eff)
+  // leaves the spare lanes and rows as dead halo
+  const int TX = g.tile[0], TY = g.tile[1];
+  const int x0 = bxi * TX - H, y0 = byi * TY - H;
+  const int gx = x0 + lane;
+  const bool in_x = gx >= 0 && gx < g.n[0];
+  const bool out_x = lane >= H && lane < H + TX;
+  const int64_t base = g.origin + (int64_t)y0 * g.sy + gx;
+
+  uint32_t mxy[FR];
+  bool inside[FR];
+#pragma unroll
+  for (int i = 0; i < FR; ++i) {
+    const int gy = y0 + i;
+    inside[i] = in_x && gy >= 0 && gy < g.n[1];
+    mxy[i] = inside[i] ? 0xFFFFFFFFu : 0u;
+  }
+  if (g.phase_limit == 1) return;
+  auto mask = [&](float v, int i) { return __uint_as_float(__float_as_uint(v) & mxy[i]); };
+
+  float u[NRW][FR];
+  float ro[NRO > 0 ? NRO : 1][FR];
+#pragma unroll
+  for (int k = 0; k < NRW; ++k)
+#pragma unroll
+    for (int i = 0; i < FR; ++i)
+      u[k][i] = (g.phase_limit == 3) ? (float)lane : (inside[i] ? f.src[k][base + (int64_t)i * g.sy] : 0.0f);
+#pragma unroll
+  for (int k = 0; k < NRO; ++k)
+#pragma unroll
+    for (int i = 0; i < FR; ++i) ro[k][i] = inside[i] ? f.ro[k][base + (int64_t)i * g.sy] : 0.0f;
+
+  // steps s = s_begin..s_end of this launch (phase limits: 2 = first step only,
+  // 3 = all but the first step)
+  const int s_begin = (g.phase_limit == 3) ? 2 : 1;
+  const int s_end = (g.phase_limit == 2) ? 1 : T;
+
+#pragma unroll
+  for (int s = 1; s <= T; ++s) {
+    if (s > s_end) break;
+    if (s < s_begin) continue;
+    if constexpr (ST == OT_J2D || ST == OT_P2D || ST == OT_G2D || ST == OT_TV2D) {
+      float nu[FR];
+#pragma unroll
+      for (int i = s; i < FR - s; ++i) {
+        const float c = u[0][i], tc = u[0][i - 1], bc = u[0][i + 1];
+        float v;
+        if constexpr (ST == OT_J2D) {
+          v = j2d(from_left(c), c, from_right(c), tc, bc);
+        } else if constexpr (ST == OT_G2D) {
+          v = g2d(from_left(c), c, from_right(c), tc, bc);
+        } else if constexpr (ST == OT_P2D) {
+          v = p2d(from_left(tc), tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
+                  from_right(bc));
+        } else {
+          const float alpha = g.coef[0] != 0.0f ? g.coef[0] : 1.1f;
+          v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc, ro[0][i], ro[1][i],
+                   alpha);
+        }
+        nu[i] = mask(v, i);
+      }
+#pragma unroll
+      for (int i = s; i < FR - s; ++i) u[0][i] = nu[i];
+    } else if constexpr (ST == OT_RICIAN2D) {
+      RicianK rk;
+      const float sigma = g.coef[0];
+      rk.sigma2 = sigma != 0.0f ? sigma * sigma : 0.0025f;
+      rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
+      rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
+      // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]
+      float gg[FR];
+#pragma unroll
+      for (int i = 2 * s - 1; i <= FR - 2 * s; ++i) {
+        const float c = u[0][i];
+        gg[i] = mask(rician_g(c, from_left(c), from_right(c), u[0][i - 1], u[0][i + 1]), i);
+      }
+      float nu[FR];
+#pragma unroll
+      for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) {
+        const float c = u[0][i];
+        const float v = rician_u(c, ro[0][i], from_left(c), from_right(c), u[0][i - 1], u[0][i + 1],
+                                 from_left(gg[i]), from_right(gg[i]), gg[i - 1], gg[i + 1], rk);
+        nu[i] = mask(v, i);
+      }
+#pragma unroll
+      for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) u[0][i] = nu[i];
+    } else {  // FDTD: u[0] = ex, u[1] = ey, u[2] = hz
+      const bool physical = (g.variant & OT_VARIANT_FDTD_PHYSICAL) != 0;
+      // Ey, Ex need Hz at (x-1, y-1): rows [s, FR-s]
+#pragma unroll
+      for (int i = s; i <= FR - s; ++i) {
+        const float hz = u[2][i];
+        const float ey = fdtd_ey(u[1][i], hz, u[2][i - 1]);
+        const float ex = fdtd_ex(u[0][i], hz, from_left(hz));
+        u[1][i] = mask(ey, i);
+        u[0][i] = mask(ex, i);
+      }
+      // Hz needs the new Ex, Ey at (x+1, y+1): rows [s, FR-1-s]
+#pragma unroll
+      for (int i = s; i <= FR - 1 - s; ++i) {
+        const float ex = u[0][i];
+        const float v = fdtd_hz(u[2][i], ex, from_right(ex), u[1][i], u[1][i + 1], physical);
+        u[2][i] = mask(v, i);
+      }
+    }
+  }
+  if (g.phase_limit != 0) return;
+
+  // write back the output tile's interior
+#pragma unroll
+  for (int i = H; i < FR - H; ++i) {
+    if (i < H + TY && out_x && inside[i]) {
+#pragma unroll
+      for (int k = 0; k < NRW; ++k) f.dst[k][base + (int64_t)i * g.sy] = u[k][i];
+    }
+  }
+}
+
+// Register budget: fields held per lane (rw + ro + Rician's g) x FR.
+template <int ST>
+constexpr int kFields = KW2<ST>::NRW + KW2<ST>::NRO + (ST == OT_RICIAN2D ? 1 : 0);
+
+template <int ST, int T, int FR>
+ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  if (prepare) return prepare_lds((const void*)&k2w<ST, T, FR>, 0);
+  const int nw = g.grid[0] * g.grid[1];
+  const int nblk = (nw + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL((k2w<ST, T, FR>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return set_error(OT_EHIP, "2D wave kernel launch failed: %s", hipGetErrorString(err));
+  return OT_OK;
+}
+
+// Footprint row counts compiled per time step count; k2w_rows() in runtime.hip
+// applies the same rule.
+template <int ST, int T>
+ot_status launch_fr(int fr, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  constexpr int H = KW2<ST>::R * T;
+  switch (fr) {
+#define FRC(V)                                                                                   \
+  case V:                                                                                        \
+    if constexpr (V > 2 * H && kFields<ST> * V <= kW2RegBudget) return launch_t<ST, T, V>(g, f, stream, prepare); \
+    break;
+    FRC(16)
+    FRC(32)
+    FRC(64)
+#undef FRC
+    default:
+      break;
+  }
+  return set_error(OT_EUNSUPPORTED, "2D wave kernel not compiled for T=%d with %d footprint rows", T, fr);
+}
+
+template <int ST>
+ot_status launch_st(int t_eff, int fr, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  switch (t_eff) {
+    case 1: return launch_fr<ST, 1>(fr, g, f, stream, prepare);
+    case 2: return launch_fr<ST, 2>(fr, g, f, stream, prepare);
+    case 3: return launch_fr<ST, 3>(fr, g, f, stream, prepare);
+    case 4: return launch_fr<ST, 4>(fr, g, f, stream, prepare);
+    case 5: return launch_fr<ST, 5>(fr, g, f, stream, prepare);
+    case 6: return launch_fr<ST, 6>(fr, g, f, stream, prepare);
+    case 7: return launch_fr<ST, 7>(fr, g, f, stream, prepare);
+    case 8: return launch_fr<ST, 8>(fr, g, f, stream, prepare);
+    default: return set_error(OT_EUNSUPPORTED, "2D wave kernels are compiled for T = 1..8 (got %d)", t_eff);
+  }
+}
+
+}  // namespace k2w
+}  // namespace ot

@@ -34,15 +34,6 @@ struct IC {
   static constexpr int value = V;
 };
 
-// lane i receives lane i-1's value (lane 0 receives 0)
-__device__ __forceinline__ float from_left(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
-}
-// lane i receives lane i+1's value (lane 63 receives 0)
-__device__ __forceinline__ float from_right(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
-}
-
 template <int ST>
 struct KW {
   static constexpr int L = (ST == OT_P3D27) ? 2 : 1;

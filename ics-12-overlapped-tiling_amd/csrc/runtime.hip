@@ -66,6 +66,17 @@ int k3w_rows(int T, const int32_t tile[3]) {
   return 0;
 }
 
+int k2w_rows(int stencil, int T, const int32_t tile[3]) {
+  const int R = (stencil == OT_RICIAN2D) ? 2 : 1;
+  const int fields = (stencil == OT_FDTD2D || stencil == OT_RICIAN2D || stencil == OT_TV2D) ? 3 : 1;
+  if (tile[0] + 2 * R * T != 64) return 0;
+  const int fr = tile[1] + 2 * R * T;
+  static const int kRows[] = {16, 32, 64};
+  for (int r : kRows)
+    if (r == fr && fr > 2 * R * T && fields * fr <= kW2RegBudget) return fr;
+  return 0;
+}
+
 ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
                     void* stream, bool prepare) {
   if (g.wave_rows > 0) {
@@ -93,7 +104,11 @@ ot_status kernel_shape(const ot_params& p, ot_derived& d) {
     shape_1d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &threads, &lds);
   } else {
     int bx = 0, by = 0;
-    if (dims == 2) {
+    if (dims == 2 && k2w_rows(p.stencil, p.time_tile, d.tile) > 0) {
+      bx = 256;  // four independent wavefronts per workgroup, no LDS
+      by = 1;
+      lds = 0;
+    } else if (dims == 2) {
       shape_2d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
     } else if (k3w_rows(p.time_tile, d.tile) > 0) {
       bx = 256;  // four independent wavefronts per workgroup, no LDS
@@ -224,7 +239,9 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   g.phase_limit = params->phase_limit;
   g.variant = params->variant;
   for (int i = 0; i < 4; ++i) g.coef[i] = params->coef[i];
-  g.wave_rows = (d.dims == 3) ? k3w_rows(params->time_tile, d.tile) : 0;
+  g.wave_rows = (d.dims == 3)   ? k3w_rows(params->time_tile, d.tile)
+                : (d.dims == 2) ? k2w_rows(params->stencil, params->time_tile, d.tile)
+                                : 0;
   // Check the instantiations this plan will launch (full T and the shortened
   // last launch) exist, and raise their LDS limits now.
   FieldPtrs f;

@@ -32,7 +32,7 @@ struct Cand {
 // Sweep ranges per dimensionality. They cover every instantiation compiled in
 // k1d.hip (ept 1..8), k2d.hip (ept 1..12), k3d_impl.hpp (T 1..8, ept 1/2/4, 8
 // for T <= 2) and k3w_impl.hpp (T 1..5, footprint rows 8..32).
-void candidates(int dims, std::vector<Cand>& out) {
+void candidates(int stencil, int dims, std::vector<Cand>& out) {
   out.clear();
   if (dims == 1) {
     for (int T = 1; T <= 8; ++T)
@@ -43,6 +43,11 @@ void candidates(int dims, std::vector<Cand>& out) {
       for (int bx : {16, 32, 64, 128})
         for (int by : {1, 2, 4, 8, 16})
           for (int e : {1, 2, 4, 8}) out.push_back({T, bx, by, 1, e});
+    // wave-per-tile kernel: x tile + 2*radius*T fills the 64 lanes
+    const int R = kTraits[stencil].radius;
+    for (int T = 1; T <= 8; ++T)
+      for (int fr : {16, 32, 64})
+        if (fr > 2 * R * T) out.push_back({T, 64 - 2 * R * T, fr - 2 * R * T, 1, 1});
   } else {
     for (int T = 1; T <= 6; ++T)
       for (int bx : {16, 32, 64})
@@ -105,7 +110,7 @@ ot_status ot_tune_space(const ot_params* base, ot_params* out, int32_t capacity,
   if (base->stencil < 0 || base->stencil >= OT_NUM_STENCILS)
     return ot::set_error(OT_EINVAL, "ot_tune_space: unknown stencil %d", base->stencil);
   std::vector<Cand> cands;
-  candidates(kTraits[base->stencil].dims, cands);
+  candidates(base->stencil, kTraits[base->stencil].dims, cands);
   int n = 0;
   for (const Cand& c : cands) {
     ot_params p = with_cand(*base, c);
@@ -136,7 +141,10 @@ ot_status ot_features(const ot_params* p, float* f) {
   f[20] = (float)log2d(p->block_y);
   f[21] = (float)log2d(p->block_z);
   f[22] = (float)log2d(p->elems_per_thread);
-  f[23] = (d.dims == 3 && ot::k3w_rows(p->time_tile, d.tile) > 0) ? 1.0f : 0.0f;
+  f[23] = ((d.dims == 3 && ot::k3w_rows(p->time_tile, d.tile) > 0) ||
+           (d.dims == 2 && ot::k2w_rows(p->stencil, p->time_tile, d.tile) > 0))
+              ? 1.0f
+              : 0.0f;
   // stencil descriptors, so that a stencil absent from the training data is
   // still placed among the others, and the analytic model as a prior
   f[24] = (float)d.dims;
@@ -282,7 +290,11 @@ ot_status ot_tune_exhaustive(const ot_params* base, const char* csv_path, int32_
                  d.real_size[0], d.real_size[1], d.real_size[2], p.time_steps, p.geometry, p.time_tile, p.block_x,
                  p.block_y, p.block_z, p.elems_per_thread, d.grid_blocks[0], d.grid_blocks[1], d.grid_blocks[2],
                  d.threads_per_block, d.lds_bytes, d.shared_size, regs,
-                 (d.dims == 3 && ot::k3w_rows(p.time_tile, d.tile) > 0) ? 1 : 0, ms == OT_OK ? t : -1.0, status);
+                 ((d.dims == 3 && ot::k3w_rows(p.time_tile, d.tile) > 0) ||
+                  (d.dims == 2 && ot::k2w_rows(p.stencil, p.time_tile, d.tile) > 0))
+                     ? 1
+                     : 0,
+                 ms == OT_OK ? t : -1.0, status);
     std::fflush(out);
     ++count;
     if (ms == OT_OK && !to && t < best_time) {

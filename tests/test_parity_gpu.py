@@ -45,6 +45,21 @@ CASES_2D = [
     (otlib.FDTD2D, (90, 61, 1), 9, 1, (16, 16, 1), 1),
     (otlib.FDTD2D, (90, 61, 1), 9, 3, (16, 8, 1), 3),
     (otlib.FDTD2D, (64, 64, 1), 16, 8, (16, 16, 1), 1),
+    # wave-per-tile kernels: x tile = 64 - 2*radius*T fills a wavefront
+    (otlib.J2D, (130, 77, 1), 9, 2, (60, 12, 1), 1),
+    (otlib.J2D, (100, 90, 1), 11, 4, (56, 24, 1), 1),
+    (otlib.J2D, (150, 140, 1), 19, 8, (48, 48, 1), 1),
+    (otlib.J2D, (70, 45, 1), 5, 1, (62, 14, 1), 1),
+    (otlib.P2D, (90, 61, 1), 7, 3, (58, 26, 1), 1),
+    (otlib.G2D, (90, 61, 1), 9, 4, (56, 8, 1), 1),
+    (otlib.TV2D, (90, 61, 1), 7, 2, (60, 28, 1), 1),
+    (otlib.TV2D, (100, 70, 1), 11, 5, (54, 6, 1), 1),
+    (otlib.FDTD2D, (90, 61, 1), 8, 3, (58, 10, 1), 1),
+    (otlib.FDTD2D, (130, 70, 1), 13, 6, (52, 20, 1), 1),
+    (otlib.FDTD2D, (70, 40, 1), 5, 1, (62, 30, 1), 1),
+    (otlib.RICIAN2D, (90, 61, 1), 7, 2, (56, 8, 1), 1),
+    (otlib.RICIAN2D, (110, 70, 1), 9, 4, (48, 16, 1), 1),
+    (otlib.RICIAN2D, (100, 50, 1), 7, 3, (52, 20, 1), 1),
 ]
 CASES_3D = [
     (otlib.J3D, (40, 37, 35), 7, 1, (16, 8, 8), 1),
