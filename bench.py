@@ -316,7 +316,7 @@ def kernel_name(w, d):
     if dims == 3:
         fam = "ot::k3w::k3w" if d.lds_bytes == 0 else "ot::k3::k3d"
     elif dims == 2:
-        fam = "ot::k2w::k2w" if d.lds_bytes == 0 else "ot::k2::k2d"
+        fam = ("ot::k2s::k2s" if d.tile[1] >= 64 else "ot::k2w::k2w") if d.lds_bytes == 0 else "ot::k2::k2d"
     else:
         fam = "ot::k1::k1d"
     return f"{fam}<{w['stencil']}>"

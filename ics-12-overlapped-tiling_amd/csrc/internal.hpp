@@ -28,7 +28,8 @@ struct LaunchGeom {
   int32_t phase_limit;
   int32_t variant;
   float coef[4];
-  int32_t wave_rows;     // footprint rows of the wave-per-tile kernel (2D, 3D), 0 = LDS kernel
+  int32_t wave_rows;     // footprint rows of the wave-per-tile kernel (2D, 3D), 0 = LDS kernel,
+                         // -1 = 2D row-streaming kernel
   int32_t reserved;
 };
 
@@ -46,6 +47,10 @@ constexpr int kW2RegBudget = 128;
 // when the tile does not fill a wavefront (x tile + 2*radius*T == 64) or has no
 // compiled row count.
 int k2w_rows(int stencil, int T, const int32_t tile[3]);
+
+// True when the row-streaming 2D kernel runs this tile: x tile + 2*radius*T
+// fills a wavefront and the y tile (the streamed row chunk) is 64 rows or more.
+bool k2s_tile(int stencil, int T, const int32_t tile[3]);
 
 // Per-launch field pointers (up to 3 rw fields, 2 ro fields).
 struct FieldPtrs {
@@ -73,6 +78,12 @@ ot_status launch_3d_p3d7(int t_eff, int ept, const LaunchGeom& g, const FieldPtr
 ot_status launch_3d_p3d27(int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_j3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_2dw_j2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_j2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_p2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_g2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_tv2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_fdtd2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_2ds_rician2d(int t_eff, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_2dw_p2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_2dw_g2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_2dw_tv2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);

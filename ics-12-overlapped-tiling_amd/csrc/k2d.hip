@@ -58,6 +58,17 @@ void shape_2d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* 
 
 ot_status launch_2d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
                     void* stream, bool prepare) {
+  if (g.wave_rows < 0) {
+    switch (stencil) {
+      case OT_J2D: return launch_2ds_j2d(t_eff, g, f, stream, prepare);
+      case OT_P2D: return launch_2ds_p2d(t_eff, g, f, stream, prepare);
+      case OT_G2D: return launch_2ds_g2d(t_eff, g, f, stream, prepare);
+      case OT_TV2D: return launch_2ds_tv2d(t_eff, g, f, stream, prepare);
+      case OT_RICIAN2D: return launch_2ds_rician2d(t_eff, g, f, stream, prepare);
+      case OT_FDTD2D: return launch_2ds_fdtd2d(t_eff, g, f, stream, prepare);
+      default: return set_error(OT_EINVAL, "stencil %d is not a 2D stencil", stencil);
+    }
+  }
   if (g.wave_rows > 0) {
     switch (stencil) {
       case OT_J2D: return launch_2dw_j2d(t_eff, g.wave_rows, g, f, stream, prepare);

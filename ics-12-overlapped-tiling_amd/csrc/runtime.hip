@@ -77,6 +77,11 @@ int k2w_rows(int stencil, int T, const int32_t tile[3]) {
   return 0;
 }
 
+bool k2s_tile(int stencil, int T, const int32_t tile[3]) {
+  const int R = (stencil == OT_RICIAN2D) ? 2 : 1;
+  return tile[0] + 2 * R * T == 64 && tile[1] >= 64;
+}
+
 ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
                     void* stream, bool prepare) {
   if (g.wave_rows > 0) {
@@ -104,7 +109,7 @@ ot_status kernel_shape(const ot_params& p, ot_derived& d) {
     shape_1d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &threads, &lds);
   } else {
     int bx = 0, by = 0;
-    if (dims == 2 && k2w_rows(p.stencil, p.time_tile, d.tile) > 0) {
+    if (dims == 2 && (k2w_rows(p.stencil, p.time_tile, d.tile) > 0 || k2s_tile(p.stencil, p.time_tile, d.tile))) {
       bx = 256;  // four independent wavefronts per workgroup, no LDS
       by = 1;
       lds = 0;
@@ -240,7 +245,9 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   g.variant = params->variant;
   for (int i = 0; i < 4; ++i) g.coef[i] = params->coef[i];
   g.wave_rows = (d.dims == 3)   ? k3w_rows(params->time_tile, d.tile)
-                : (d.dims == 2) ? k2w_rows(params->stencil, params->time_tile, d.tile)
+                : (d.dims == 2) ? (k2s_tile(params->stencil, params->time_tile, d.tile)
+                                       ? -1
+                                       : k2w_rows(params->stencil, params->time_tile, d.tile))
                                 : 0;
   // Check the instantiations this plan will launch (full T and the shortened
   // last launch) exist, and raise their LDS limits now.

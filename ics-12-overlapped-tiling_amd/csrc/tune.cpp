@@ -48,6 +48,10 @@ void candidates(int stencil, int dims, std::vector<Cand>& out) {
     for (int T = 1; T <= 8; ++T)
       for (int fr : {16, 32, 64})
         if (fr > 2 * R * T) out.push_back({T, 64 - 2 * R * T, fr - 2 * R * T, 1, 1});
+    // row-streaming kernel: the same x tiles streaming chunks of 64..512 rows
+    for (int T = 1; T <= 8; ++T)
+      if (64 - 2 * R * T > 0)
+        for (int rows : {64, 128, 256, 512}) out.push_back({T, 64 - 2 * R * T, rows, 1, 1});
   } else {
     for (int T = 1; T <= 6; ++T)
       for (int bx : {16, 32, 64})
@@ -142,7 +146,8 @@ ot_status ot_features(const ot_params* p, float* f) {
   f[21] = (float)log2d(p->block_z);
   f[22] = (float)log2d(p->elems_per_thread);
   f[23] = ((d.dims == 3 && ot::k3w_rows(p->time_tile, d.tile) > 0) ||
-           (d.dims == 2 && ot::k2w_rows(p->stencil, p->time_tile, d.tile) > 0))
+           (d.dims == 2 && (ot::k2w_rows(p->stencil, p->time_tile, d.tile) > 0 ||
+                            ot::k2s_tile(p->stencil, p->time_tile, d.tile))))
               ? 1.0f
               : 0.0f;
   // stencil descriptors, so that a stencil absent from the training data is
@@ -291,7 +296,8 @@ ot_status ot_tune_exhaustive(const ot_params* base, const char* csv_path, int32_
                  p.block_y, p.block_z, p.elems_per_thread, d.grid_blocks[0], d.grid_blocks[1], d.grid_blocks[2],
                  d.threads_per_block, d.lds_bytes, d.shared_size, regs,
                  ((d.dims == 3 && ot::k3w_rows(p.time_tile, d.tile) > 0) ||
-                  (d.dims == 2 && ot::k2w_rows(p.stencil, p.time_tile, d.tile) > 0))
+                  (d.dims == 2 && (ot::k2w_rows(p.stencil, p.time_tile, d.tile) > 0 ||
+                                   ot::k2s_tile(p.stencil, p.time_tile, d.tile))))
                      ? 1
                      : 0,
                  ms == OT_OK ? t : -1.0, status);

@@ -60,6 +60,20 @@ CASES_2D = [
     (otlib.RICIAN2D, (90, 61, 1), 7, 2, (56, 8, 1), 1),
     (otlib.RICIAN2D, (110, 70, 1), 9, 4, (48, 16, 1), 1),
     (otlib.RICIAN2D, (100, 50, 1), 7, 3, (52, 20, 1), 1),
+    # row-streaming kernels: the same x tiles, y tile = streamed chunk of >= 64 rows
+    (otlib.J2D, (200, 150, 1), 9, 2, (60, 64, 1), 1),
+    (otlib.J2D, (130, 230, 1), 19, 8, (48, 100, 1), 1),
+    (otlib.J2D, (70, 40, 1), 5, 1, (62, 64, 1), 1),
+    (otlib.P2D, (130, 140, 1), 7, 3, (58, 70, 1), 1),
+    (otlib.G2D, (90, 130, 1), 9, 4, (56, 64, 1), 1),
+    (otlib.TV2D, (100, 150, 1), 7, 3, (58, 80, 1), 1),
+    (otlib.TV2D, (120, 70, 1), 11, 5, (54, 64, 1), 1),
+    (otlib.FDTD2D, (130, 170, 1), 8, 3, (58, 64, 1), 1),
+    (otlib.FDTD2D, (110, 210, 1), 13, 6, (52, 100, 1), 1),
+    (otlib.FDTD2D, (70, 90, 1), 5, 1, (62, 64, 1), 1),
+    (otlib.RICIAN2D, (110, 150, 1), 7, 2, (56, 64, 1), 1),
+    (otlib.RICIAN2D, (100, 200, 1), 9, 4, (48, 96, 1), 1),
+    (otlib.RICIAN2D, (90, 64, 1), 7, 3, (52, 64, 1), 1),
 ]
 CASES_3D = [
     (otlib.J3D, (40, 37, 35), 7, 1, (16, 8, 8), 1),
