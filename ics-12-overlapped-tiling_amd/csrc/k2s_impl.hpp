@@ -22,6 +22,10 @@
 //     never reach an output row), which keeps the queues uniform.
 //   * Output rows are stored one iteration later (so load waits never wait for
 //     stores).
+//   * The queues are rings whose slots follow the (unrolled) iteration phase,
+//     so advancing a level moves no registers.
+//   * A chunk whose whole dependence window lies inside the domain runs a copy
+//     of the loop without the per-point Dirichlet masks.
 #pragma once
 #include "internal.hpp"
 #include "device.hpp"
@@ -47,11 +51,23 @@ struct KS {
 };
 
 constexpr int kWaves = 4;
-constexpr int PD = 4;  // rows prefetched ahead
-constexpr int U = 4;   // iteration unroll (a multiple of PD)
+constexpr int PD = 4;   // rows prefetched ahead
+constexpr int U = 12;   // iteration unroll: a multiple of every ring depth below
+constexpr int RQR = 12; // ring depth of the read-only rows (TV: T+1, Rician: 2T+1 rows needed)
 
-template <int ST, int T>
-__global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
+// Ring slot holding the row produced j iterations before the iteration of phase PH.
+template <int D>
+__device__ constexpr int slot(int PH, int j) {
+  return (PH + 4 * D - j) % D;
+}
+
+template <int ST>
+constexpr int max_t() {
+  return ST == OT_RICIAN2D ? (RQR - 1) / 2 : 8;
+}
+
+template <int ST, int T, bool CHK>
+__device__ __forceinline__ void run(const LaunchGeom& g, const FieldPtrs& f, const int bxi, const int byi) {
   constexpr int R = KS<ST>::R;
   constexpr int NRW = KS<ST>::NRW;
   constexpr int NRO = KS<ST>::NRO;
@@ -59,11 +75,6 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
   constexpr int H = R * T;
   constexpr int LAG = R * T;  // row lag of the last level behind the load front
   const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, (int)gridDim.x) * kWaves +
-                                                 (int)(threadIdx.x >> 6));
-  const int nw = g.grid[0] * g.grid[1];
-  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
-  const int bxi = wid % g.grid[0], byi = wid / g.grid[0];
   const int TX = g.tile[0], TY = g.tile[1];
   const int x0 = bxi * TX - H, y0 = byi * TY;
   const int gx = x0 + lane;
@@ -91,7 +102,6 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
   // lane term of a byte offset: out-of-domain lanes get `range` (read 0, store dropped)
   const uint32_t lane_ld = in_x ? (uint32_t)gx * 4u : range;
   const uint32_t lane_st = (in_x && out_x) ? (uint32_t)gx * 4u : range;
-  if (g.phase_limit == 1) return;
   const bool do_loads = g.phase_limit != 3;
   const bool do_store = g.phase_limit == 0;
   const int s_end = (g.phase_limit == 2) ? 1 : T;
@@ -108,9 +118,12 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
     for (int q = 0; q < NF; ++q)
       pf[SL][q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs[q], vo, 0, 0));
   };
-  // Dirichlet 0: values at points outside the domain are forced to +0
-  auto mrow = [&](int p) -> bool { return p >= 0 && p < g.n[1]; };
-  auto masked = [&](float v, bool row_in) { return (row_in && in_x) ? v : 0.0f; };
+  // Dirichlet 0: values at points outside the domain are forced to +0 (only
+  // chunks whose dependence window leaves the domain need it)
+  auto masked = [&](float v, int p) {
+    if constexpr (CHK) return (p >= 0 && p < g.n[1] && in_x) ? v : 0.0f;
+    else return v;
+  };
 
   // deferred output row
   float outv[NRW];
@@ -125,17 +138,12 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[q]), rd[q], vo, 0, 0);
   };
 
-  // ---- per-stencil register queues (index 0 = newest row) ----
-  // single-field: lv[s][j] = level s at row (newest - j)
-  float lv[T + 1][3];
-  // TV / Rician read-only rows: rq[q][j] = row k - j
-  constexpr int RQ = (ST == OT_RICIAN2D) ? 2 * T + 1 : T + 1;
-  float rq[NRO > 0 ? NRO : 1][RQ];
-  // FDTD: fe[s][field][j], field 0 = ex, 1 = ey, 2 = hz; depth 2
-  float fe[T + 1][3][2];
-  // Rician: uq[s][j] depth 4, gq[s][j] depth 3
-  float uq[T + 1][4];
-  float gq[T + 1][3];
+  // ---- per-stencil register rings ----
+  float lv[T + 1][3];                       // single-field levels
+  float rq[NRO > 0 ? NRO : 1][RQR];         // TV / Rician read-only rows
+  float fe[T + 1][3][2];                    // FDTD ex, ey, hz per level
+  float uq[T + 1][4];                       // Rician u per level
+  float gq[T + 1][3];                       // Rician g per level
 #pragma unroll
   for (int s = 0; s <= T; ++s) {
 #pragma unroll
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
 #pragma unroll
   for (int q = 0; q < (NRO > 0 ? NRO : 1); ++q)
 #pragma unroll
-    for (int j = 0; j < RQ; ++j) rq[q][j] = 0.0f;
+    for (int j = 0; j < RQR; ++j) rq[q][j] = 0.0f;
 
   fetch(kfirst, IC<0>{});
   fetch(kfirst + 1, IC<1>{});
@@ -166,7 +174,8 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
   const bool physical = (g.variant & OT_VARIANT_FDTD_PHYSICAL) != 0;
 
   auto iter = [&](auto phc, const int k) {
-    constexpr int PS = decltype(phc)::value % PD;
+    constexpr int PH = decltype(phc)::value;
+    constexpr int PS = PH % PD;
     flush();
     out_p = -1;
     // level 0 <- row k
@@ -175,21 +184,17 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
     for (int q = 0; q < NF; ++q) row[q] = pf[PS][q];
     fetch(k + PD, IC<PS>{});
 #pragma unroll
-    for (int q = 0; q < NRO; ++q) {
-#pragma unroll
-      for (int j = RQ - 1; j > 0; --j) rq[q][j] = rq[q][j - 1];
-      rq[q][0] = row[NRW + q];
-    }
+    for (int q = 0; q < NRO; ++q) rq[q][slot<RQR>(PH, 0)] = row[NRW + q];
 
     if constexpr (ST == OT_J2D || ST == OT_P2D || ST == OT_G2D || ST == OT_TV2D) {
-      lv[0][2] = lv[0][1];
-      lv[0][1] = lv[0][0];
-      lv[0][0] = row[0];
+      constexpr int N0 = slot<3>(PH, 0), N1 = slot<3>(PH, 1), N2 = slot<3>(PH, 2);
+      lv[0][N0] = row[0];
 #pragma unroll
       for (int s = 1; s <= T; ++s) {
         if (s > s_end) break;
+        // level s at row p = k - s from level s-1 rows p+1 (newest), p, p-1
         const int p = k - s;
-        const float c = lv[s - 1][1], tc = lv[s - 1][2], bc = lv[s - 1][0];
+        const float c = lv[s - 1][N1], tc = lv[s - 1][N2], bc = lv[s - 1][N0];
         float v;
         if constexpr (ST == OT_J2D) {
           v = j2d(from_left(c), c, from_right(c), tc, bc);
@@ -199,78 +204,67 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
           v = p2d(from_left(tc), tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
                   from_right(bc));
         } else {
-          v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc, rq[0][s], rq[1][s],
-                   alpha);
+          v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
+                   rq[0][slot<RQR>(PH, s)], rq[1][slot<RQR>(PH, s)], alpha);
         }
-        v = masked(v, mrow(p));
-        lv[s][2] = lv[s][1];
-        lv[s][1] = lv[s][0];
-        lv[s][0] = v;
+        v = masked(v, p);
+        lv[s][N0] = v;
         if (s == T && do_store && p >= y0 && p < y0 + TY && p < g.n[1]) {
           outv[0] = v;
           out_p = p;
         }
       }
     } else if constexpr (ST == OT_RICIAN2D) {
-#pragma unroll
-      for (int j = 3; j > 0; --j) uq[0][j] = uq[0][j - 1];
-      uq[0][0] = row[0];
+      uq[0][slot<4>(PH, 0)] = row[0];
 #pragma unroll
       for (int s = 1; s <= T; ++s) {
         if (s > s_end) break;
-        // g of level s at row k - 2s + 1 from u of level s-1 (rows +1, 0, -1 = uq[0], [1], [2])
+        // g of level s at row k - 2s + 1 from u of level s-1 rows +1, 0, -1 (ring ages 0, 1, 2)
         const int pg = k - 2 * s + 1;
         {
-          const float c = uq[s - 1][1];
-          const float gv = rician_g(c, from_left(c), from_right(c), uq[s - 1][2], uq[s - 1][0]);
-          gq[s][2] = gq[s][1];
-          gq[s][1] = gq[s][0];
-          gq[s][0] = masked(gv, mrow(pg));
+          const float c = uq[s - 1][slot<4>(PH, 1)];
+          const float gv = rician_g(c, from_left(c), from_right(c), uq[s - 1][slot<4>(PH, 2)],
+                                    uq[s - 1][slot<4>(PH, 0)]);
+          gq[s][slot<3>(PH, 0)] = masked(gv, pg);
         }
-        // u of level s at row k - 2s from u of level s-1 (rows +1, 0, -1 = uq[1], [2], [3]) and g
+        // u of level s at row k - 2s from u of level s-1 rows +1, 0, -1 (ages 1, 2, 3) and g
         const int pu = k - 2 * s;
-        const float c = uq[s - 1][2];
-        const float gc = gq[s][1];
-        const float v = masked(rician_u(c, rq[0][2 * s], from_left(c), from_right(c), uq[s - 1][3], uq[s - 1][1],
-                                        from_left(gc), from_right(gc), gq[s][2], gq[s][0], rk),
-                               mrow(pu));
-#pragma unroll
-        for (int j = 3; j > 0; --j) uq[s][j] = uq[s][j - 1];
-        uq[s][0] = v;
+        const float c = uq[s - 1][slot<4>(PH, 2)];
+        const float gc = gq[s][slot<3>(PH, 1)];
+        const float v = masked(rician_u(c, rq[0][slot<RQR>(PH, 2 * s)], from_left(c), from_right(c),
+                                        uq[s - 1][slot<4>(PH, 3)], uq[s - 1][slot<4>(PH, 1)], from_left(gc),
+                                        from_right(gc), gq[s][slot<3>(PH, 2)], gq[s][slot<3>(PH, 0)], rk),
+                               pu);
+        uq[s][slot<4>(PH, 0)] = v;
         if (s == T && do_store && pu >= y0 && pu < y0 + TY && pu < g.n[1]) {
           outv[0] = v;
           out_p = pu;
         }
       }
     } else {  // FDTD
+      constexpr int E0 = slot<2>(PH, 0), E1 = slot<2>(PH, 1);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        fe[0][a][1] = fe[0][a][0];
-        fe[0][a][0] = row[a];
-      }
+      for (int a = 0; a < 3; ++a) fe[0][a][E0] = row[a];
 #pragma unroll
       for (int s = 1; s <= T; ++s) {
         if (s > s_end) break;
-        // Ey, Ex of level s at row r = k - s + 1: level s-1 Hz rows r (index 0), r - 1 (index 1);
-        // level s-1 Ex, Ey at row r: index 0 for the loaded level, 1 above it
+        // Ey, Ex of level s at row r = k - s + 1: level s-1 Hz rows r (newest), r - 1;
+        // level s-1 Ex, Ey at row r: the newest row of the loaded level, one older above it
         const int r = k - s + 1;
-        const int ie = (s == 1) ? 0 : 1;
-        const float hz = fe[s - 1][2][0];
-        const float ey = masked(fdtd_ey(fe[s - 1][1][ie], hz, fe[s - 1][2][1]), mrow(r));
-        const float ex = masked(fdtd_ex(fe[s - 1][0][ie], hz, from_left(hz)), mrow(r));
-        fe[s][0][1] = fe[s][0][0];
-        fe[s][0][0] = ex;
-        fe[s][1][1] = fe[s][1][0];
-        fe[s][1][0] = ey;
-        // Hz of level s at row r - 1 from the new Ex, Ey (rows r - 1 = index 1, r = index 0)
-        const float exc = fe[s][0][1];
-        const float hzv = masked(fdtd_hz(fe[s - 1][2][1], exc, from_right(exc), fe[s][1][1], fe[s][1][0], physical),
-                                 mrow(r - 1));
-        fe[s][2][1] = fe[s][2][0];
-        fe[s][2][0] = hzv;
+        const int ie = (s == 1) ? E0 : E1;
+        const float hz = fe[s - 1][2][E0];
+        const float ey = masked(fdtd_ey(fe[s - 1][1][ie], hz, fe[s - 1][2][E1]), r);
+        const float ex = masked(fdtd_ex(fe[s - 1][0][ie], hz, from_left(hz)), r);
+        fe[s][0][E0] = ex;
+        fe[s][1][E0] = ey;
+        // Hz of level s at row r - 1 from the new Ex, Ey (rows r - 1 = older slot, r = newest)
+        const float exc = fe[s][0][E1];
+        const float hzv =
+            masked(fdtd_hz(fe[s - 1][2][E1], exc, from_right(exc), fe[s][1][E1], ey, physical), r - 1);
+        fe[s][2][E0] = hzv;
         if (s == T && do_store && r - 1 >= y0 && r - 1 < y0 + TY && r - 1 < g.n[1]) {
-          outv[0] = fe[s][0][1];
-          outv[1] = fe[s][1][1];
+          outv[0] = exc;
+          outv[1] = fe[s][1][E1];
           outv[2] = hzv;
           out_p = r - 1;
         }
@@ -284,26 +278,66 @@ __global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
     iter(IC<1>{}, k + 1);
     iter(IC<2>{}, k + 2);
     iter(IC<3>{}, k + 3);
+    iter(IC<4>{}, k + 4);
+    iter(IC<5>{}, k + 5);
+    iter(IC<6>{}, k + 6);
+    iter(IC<7>{}, k + 7);
+    iter(IC<8>{}, k + 8);
+    iter(IC<9>{}, k + 9);
+    iter(IC<10>{}, k + 10);
+    iter(IC<11>{}, k + 11);
   }
   if (k <= klast) iter(IC<0>{}, k);
   if (k + 1 <= klast) iter(IC<1>{}, k + 1);
   if (k + 2 <= klast) iter(IC<2>{}, k + 2);
+  if (k + 3 <= klast) iter(IC<3>{}, k + 3);
+  if (k + 4 <= klast) iter(IC<4>{}, k + 4);
+  if (k + 5 <= klast) iter(IC<5>{}, k + 5);
+  if (k + 6 <= klast) iter(IC<6>{}, k + 6);
+  if (k + 7 <= klast) iter(IC<7>{}, k + 7);
+  if (k + 8 <= klast) iter(IC<8>{}, k + 8);
+  if (k + 9 <= klast) iter(IC<9>{}, k + 9);
+  if (k + 10 <= klast) iter(IC<10>{}, k + 10);
   flush();
+}
+
+template <int ST, int T>
+__global__ void __launch_bounds__(kWaves * 64) k2s(LaunchGeom g, FieldPtrs f) {
+  constexpr int H = KS<ST>::R * T;
+  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, (int)gridDim.x) * kWaves +
+                                                 (int)(threadIdx.x >> 6));
+  const int nw = g.grid[0] * g.grid[1];
+  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
+  if (g.phase_limit == 1) return;
+  const int bxi = wid % g.grid[0], byi = wid / g.grid[0];
+  const int x0 = bxi * g.tile[0] - H, y0 = byi * g.tile[1];
+  // every point of the chunk's dependence window inside the domain: no masks
+  const bool interior = x0 >= 0 && x0 + 63 < g.n[0] && y0 - H >= 0 && y0 + g.tile[1] - 1 + H < g.n[1];
+  if (interior)
+    run<ST, T, false>(g, f, bxi, byi);
+  else
+    run<ST, T, true>(g, f, bxi, byi);
 }
 
 template <int ST, int T>
 ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
   constexpr int H = KS<ST>::R * T;
+  if constexpr (T > max_t<ST>()) {
+    return set_error(OT_EUNSUPPORTED, "2D row-streaming kernel not compiled for T=%d of this stencil", T);
+  }
   if ((int64_t)(g.tile[1] + 2 * H + PD + 1) * g.sy * 4 >= (1LL << 30))
     return set_error(OT_EUNSUPPORTED, "2D row chunk of %d rows x %lld bytes too large", g.tile[1],
                      (long long)(g.sy * 4));
-  if (prepare) return prepare_lds((const void*)&k2s<ST, T>, 0);
-  const int nw = g.grid[0] * g.grid[1];
-  const int nblk = (nw + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL((k2s<ST, T>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f);
-  const hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return set_error(OT_EHIP, "2D row-streaming kernel launch failed: %s", hipGetErrorString(err));
-  return OT_OK;
+  if constexpr (T <= max_t<ST>()) {
+    if (prepare) return prepare_lds((const void*)&k2s<ST, T>, 0);
+    const int nw = g.grid[0] * g.grid[1];
+    const int nblk = (nw + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL((k2s<ST, T>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f);
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+      return set_error(OT_EHIP, "2D row-streaming kernel launch failed: %s", hipGetErrorString(err));
+    return OT_OK;
+  }
 }
 
 template <int ST>

@@ -79,6 +79,8 @@ int k2w_rows(int stencil, int T, const int32_t tile[3]) {
 
 bool k2s_tile(int stencil, int T, const int32_t tile[3]) {
   const int R = (stencil == OT_RICIAN2D) ? 2 : 1;
+  // Rician keeps 2T+1 rows of f in a 12-slot ring: compiled for T <= 5
+  if (stencil == OT_RICIAN2D && T > 5) return false;
   return tile[0] + 2 * R * T == 64 && tile[1] >= 64;
 }
 

@@ -49,9 +49,8 @@ void candidates(int stencil, int dims, std::vector<Cand>& out) {
       for (int fr : {16, 32, 64})
         if (fr > 2 * R * T) out.push_back({T, 64 - 2 * R * T, fr - 2 * R * T, 1, 1});
     // row-streaming kernel: the same x tiles streaming chunks of 64..512 rows
-    for (int T = 1; T <= 8; ++T)
-      if (64 - 2 * R * T > 0)
-        for (int rows : {64, 128, 256, 512}) out.push_back({T, 64 - 2 * R * T, rows, 1, 1});
+    for (int T = 1; T <= (stencil == OT_RICIAN2D ? 5 : 8); ++T)
+      for (int rows : {64, 128, 256, 512}) out.push_back({T, 64 - 2 * R * T, rows, 1, 1});
   } else {
     for (int T = 1; T <= 6; ++T)
       for (int bx : {16, 32, 64})
