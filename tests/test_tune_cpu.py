@@ -109,3 +109,23 @@ def test_analytic_scaling_in_steps_and_launch_penalty(otlib):
 def test_tune_entry_points_reject_bad_arguments(otlib):
     with pytest.raises(otlib.OtError):
         ottune.analytic_seconds(_p("jacobi-2d", (64, 64, 1), 4, 1, 4096, 4, 1, 1), 32)
+
+
+def test_space_holds_all_2d_kernel_families(otlib):
+    """2D spaces carry the LDS tiles, the wave-per-tile kernel (x tile + 2rT = 64,
+    footprint rows 16/32/64) and the row-streaming kernel (chunks of >= 64 rows);
+    Rician's streaming kernel stops at T = 5."""
+    for name, R in (("jacobi-2d", 1), ("fdtd-2d", 1), ("rician-denoise-2d", 2)):
+        sp = ottune.space(_p(name, (2048, 2048, 1), 24, 1, 16, 4, 1, 1))
+        keys = [(q.time_tile, q.block_x, q.block_y, q.elems_per_thread) for q in sp]
+        wave = [k for k in keys if k[1] + 2 * R * k[0] == 64]
+        assert any(k[2] >= 64 for k in wave) and any(k[2] < 64 for k in wave), name
+        assert any(k[1] + 2 * R * k[0] != 64 for k in keys), name
+        for q in sp:
+            d = otlib.derive(q)
+            is_wave = q.block_x + 2 * R * q.time_tile == 64 and (q.block_y >= 64 or q.block_y + 2 * R * q.time_tile
+                                                                in (16, 32, 64))
+            if is_wave and not (name == "rician-denoise-2d" and q.block_y >= 64 and q.time_tile > 5):
+                assert d.lds_bytes == 0 and d.threads_per_block == 256, (name, q.time_tile, q.block_y)
+        if name == "rician-denoise-2d":
+            assert not [k for k in wave if k[2] >= 64 and k[0] > 5]
