@@ -403,6 +403,7 @@ ot_status ot_batch_run(ot_batch* b, float* const* rw_a, float* const* rw_b, cons
 
 void ot_batch_destroy(ot_batch* b) {
   if (!b) return;
+  if (b->own_stream) (void)hipStreamSynchronize(b->own_stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);

@@ -160,15 +160,25 @@ extern "C" ot_status ot_naive_run(const ot_params* params, float* const* rw_a, f
   hipStream_t s = (hipStream_t)stream;
   const int64_t total = (int64_t)g.n[0] * g.n[1] * g.n[2];
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
-  // Rician keeps g of every point between its two sub-stages.
-  float* gbuf = nullptr;
+  // Rician keeps g of every point between its two sub-stages; the scratch
+  // buffer and the events are released on every return path
+  struct Scratch {
+    float* gbuf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scratch() {
+      if (gbuf) (void)hipFree(gbuf);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } sc;
   if (p.stencil == OT_RICIAN2D) {
-    OT_HIP(hipMalloc(&gbuf, (size_t)d.field_elems * sizeof(float)));
-    OT_HIP(hipMemset(gbuf, 0, (size_t)d.field_elems * sizeof(float)));
+    OT_HIP(hipMalloc(&sc.gbuf, (size_t)d.field_elems * sizeof(float)));
+    OT_HIP(hipMemset(sc.gbuf, 0, (size_t)d.field_elems * sizeof(float)));
   }
-  hipEvent_t e0, e1;
-  OT_HIP(hipEventCreate(&e0));
-  OT_HIP(hipEventCreate(&e1));
+  float* const gbuf = sc.gbuf;
+  OT_HIP(hipEventCreate(&sc.e0));
+  OT_HIP(hipEventCreate(&sc.e1));
+  const hipEvent_t e0 = sc.e0, e1 = sc.e1;
   OT_HIP(hipEventRecord(e0, s));
   int final_buffer = 0;
   for (int t = 0; t < p.time_steps; ++t) {
@@ -212,9 +222,6 @@ extern "C" ot_status ot_naive_run(const ot_params* params, float* const* rw_a, f
   OT_HIP(hipEventSynchronize(e1));
   float ms = 0.0f;
   OT_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (gbuf) OT_HIP(hipFree(gbuf));
   if (timing) {
     timing->wall_s = 0.0;
     timing->event_s = ms * 1e-3;

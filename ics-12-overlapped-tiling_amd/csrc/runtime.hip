@@ -278,6 +278,8 @@ ot_status ot_plan_derived(const ot_plan* plan, ot_derived* derived) {
 
 void ot_plan_destroy(ot_plan* plan) {
   if (!plan) return;
+  // solves enqueued with OT_RUN_ASYNC may still run on the plan's stream
+  if (plan->own_stream) (void)hipStreamSynchronize(plan->own_stream);
   if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
   if (plan->ev0) (void)hipEventDestroy(plan->ev0);
   if (plan->ev1) (void)hipEventDestroy(plan->ev1);
