@@ -74,12 +74,16 @@ def base_params(name, n3):
                              y=4, z=4, e=1)
 
 
-def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float):
+def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float, stencils=None):
+    """Exhaustive sweeps of the shapes drawn for `seed`, appended to
+    out_dir/timings.csv. `stencils` (names) restricts the sweep to those
+    stencils' shapes; the draw itself is unchanged, so a restricted run
+    measures exactly the shapes a full run would for them."""
     os.makedirs(out_dir, exist_ok=True)
     csv_path = os.path.join(out_dir, "timings.csv")
     sweeps_path = os.path.join(out_dir, "sweeps.json")
     sweeps = json.load(open(sweeps_path)) if os.path.exists(sweeps_path) else {}
-    todo = shapes(per_stencil, seed)
+    todo = [(name, n3) for name, n3 in shapes(per_stencil, seed) if not stencils or name in stencils]
     for i, (name, n3) in enumerate(todo):
         key = shape_key(name, n3)
         t0 = time.perf_counter()
@@ -263,6 +267,7 @@ def main():
     c.add_argument("--per-stencil", type=int, default=5)
     c.add_argument("--seed", type=int, default=2024)
     c.add_argument("--time-limit", type=float, default=0.05, help="watchdog: seconds per solve")
+    c.add_argument("--stencils", default="", help="comma-separated stencil names to sweep (default: all)")
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--model", required=True)
@@ -283,7 +288,7 @@ def main():
     lo.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
     if a.cmd == "collect":
-        collect(a.out, a.per_stencil, a.seed, a.time_limit)
+        collect(a.out, a.per_stencil, a.seed, a.time_limit, [x for x in a.stencils.split(",") if x])
     elif a.cmd == "train":
         train(a.data, a.model, a.epochs, a.batch, a.lr, a.seed)
     elif a.cmd == "evaluate":

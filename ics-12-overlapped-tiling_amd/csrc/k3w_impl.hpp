@@ -17,6 +17,9 @@
 //     is prefetched PD planes ahead through buffer loads whose out-of-range
 //     lane offset reads 0 outside the domain; results are stored through
 //     buffer stores one iteration later (so load waits never wait for stores).
+//   * Chunks away from the z faces run their middle iterations as unrolled
+//     blocks without the per-stage range and domain checks (the checks are
+//     scalar work that each wave issues in order with its vector work).
 // Out-of-domain points are forced to +0 at every level (Dirichlet 0).
 #pragma once
 #include "internal.hpp"
@@ -53,6 +56,7 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   constexpr int NQ = KW<ST>::NQ;
   constexpr int U = KW<ST>::U;
   constexpr int PD = KW<ST>::PD;
+  constexpr bool kSteady = L == 1 && !RHS;  // see the steady state below
   static_assert(U % PD == 0 && U % NQ == 0, "unroll must cycle the prefetch ring and the queue");
   static_assert(FR > 2 * T, "footprint rows must exceed the halo");
   const int lane = threadIdx.x & 63;
@@ -129,42 +133,47 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   const int s_end = (g.phase_limit == 2) ? 1 : T;
 
   float pf[PD][FR], pfr[PD][FR];
-  auto fetch = [&](int k, auto slotc) {
+  // FULL = 0: a steady-state iteration of an interior chunk (see below), whose
+  // plane k is loaded exactly when k <= kload_end
+  auto fetch = [&](int k, auto slotc, auto fullc) __attribute__((always_inline)) {
     constexpr int SL = decltype(slotc)::value;
-    const bool live = do_loads && k >= 0 && k < g.n[2] && k <= kload_end;
-    const uint32_t poff = (uint32_t)(live ? k - (z0 - T) : 0) * plane_bytes;
+    constexpr bool FULL = decltype(fullc)::value != 0;
+    const bool live = FULL ? (do_loads && k >= 0 && k < g.n[2] && k <= kload_end) : (k <= kload_end);
+    // plane term = range for a plane that is not loaded: every offset below
+    // then reaches the range (the three terms stay under 3 GiB, no wrap)
+    const uint32_t pterm = live ? (uint32_t)(k - (z0 - T)) * plane_bytes : range;
 #pragma unroll
     for (int i = 0; i < FR; ++i) {
       // the plane term changes every iteration, which keeps the compiler from
       // hoisting FR per-row sums into VGPRs
-      const int vo = (int)(lane_ld + (live ? row_ld[i] + poff : range));
+      const int vo = (int)(lane_ld + (row_ld[i] + pterm));
       pf[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, vo, 0, 0));
       if constexpr (RHS)
         pfr[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rro, vo, 0, 0));
     }
   };
-  fetch(kstart, IC<0>{});
-  if constexpr (PD > 1) fetch(kstart + 1, IC<1>{});
-  if constexpr (PD > 2) fetch(kstart + 2, IC<2>{});
-  if constexpr (PD > 3) fetch(kstart + 3, IC<3>{});
-  if constexpr (PD > 4) fetch(kstart + 4, IC<4>{});
-  if constexpr (PD > 5) fetch(kstart + 5, IC<5>{});
+  fetch(kstart, IC<0>{}, IC<1>{});
+  if constexpr (PD > 1) fetch(kstart + 1, IC<1>{}, IC<1>{});
+  if constexpr (PD > 2) fetch(kstart + 2, IC<2>{}, IC<1>{});
+  if constexpr (PD > 3) fetch(kstart + 3, IC<3>{}, IC<1>{});
+  if constexpr (PD > 4) fetch(kstart + 4, IC<4>{}, IC<1>{});
+  if constexpr (PD > 5) fetch(kstart + 5, IC<5>{}, IC<1>{});
 
   float outv[FR];
 #pragma unroll
   for (int i = 0; i < FR; ++i) outv[i] = 0.0f;
   int out_p = -1;
-  auto flush = [&]() {
-    const bool live = out_p >= 0;
-    const uint32_t poff = (uint32_t)(live ? out_p - (z0 - T) : 0) * plane_bytes;
+  auto flush = [&]() __attribute__((always_inline)) {
+    const uint32_t pterm = out_p >= 0 ? (uint32_t)(out_p - (z0 - T)) * plane_bytes : range;
 #pragma unroll
     for (int i = T; i < FR - T; ++i)
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[i]), rdst,
-                                            (int)(lane_st + (live ? row_st[i] + poff : range)), 0, 0);
+                                            (int)(lane_st + (row_st[i] + pterm)), 0, 0);
   };
 
-  auto iter = [&](auto phc, const int k) {
+  auto iter = [&](auto phc, auto fullc, const int k) __attribute__((always_inline)) {
     constexpr int PH = decltype(phc)::value;
+    constexpr bool FULL = decltype(fullc)::value != 0;
     constexpr int PS = PH % PD;
     // register queue slots: the plane produced at iteration j sits in slot j mod NQ
     constexpr int QN = PH % NQ;
@@ -184,22 +193,24 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
     }
     flush();
     out_p = -1;
-    fetch(k + PD, IC<PS>{});
+    fetch(k + PD, IC<PS>{}, fullc);
 
 #pragma unroll
     for (int s = 1; s <= T; ++s) {
-      if (s > s_end) break;
       const int p = k - L * s;
-      const int lo = z0 - (T - s), hi = z0 + TZ - 1 + (T - s);
-      if (p < lo || p > hi) continue;
-      if (p < 0 || p >= g.n[2]) {
-        // plane outside the domain: level s is 0 there (wave-uniform branch,
-        // so the per-point mask below only carries the x/y validity)
-        if (s < T) {
+      if constexpr (FULL) {
+        if (s > s_end) break;
+        const int lo = z0 - (T - s), hi = z0 + TZ - 1 + (T - s);
+        if (p < lo || p > hi) continue;
+        if (p < 0 || p >= g.n[2]) {
+          // plane outside the domain: level s is 0 there (wave-uniform branch,
+          // so the per-point mask below only carries the x/y validity)
+          if (s < T) {
 #pragma unroll
-          for (int i = s; i < FR - s; ++i) q[s][QN][i] = 0.0f;
+            for (int i = s; i < FR - s; ++i) q[s][QN][i] = 0.0f;
+          }
+          continue;
         }
-        continue;
       }
 #pragma unroll
       for (int i = s; i < FR - s; ++i) {
@@ -238,27 +249,50 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
         if (s < T) q[s][QN][i] = nv;
         else outv[i] = nv;
       }
-      if (s == T && do_store && p >= z0 && p < zout_end) out_p = p;
+      if constexpr (FULL) {
+        if (s == T && do_store && p >= z0 && p < zout_end) out_p = p;
+      } else if (s == T) {
+        out_p = p;
+      }
     }
+    // keep the scheduler from interleaving consecutive steady iterations (it
+    // would hold several iterations' values live at once)
+    if constexpr (!FULL) __builtin_amdgcn_sched_barrier(0);
   };
 
-  int k = kstart;
-  for (; k + U - 1 <= kend; k += U) {
-    iter(IC<0>{}, k);
-    iter(IC<1>{}, k + 1);
-    iter(IC<2>{}, k + 2);
-    iter(IC<3>{}, k + 3);
+  // Steady state (7-point stencils without a right-hand side; the others keep
+  // one checked loop, which holds fewer registers). In a chunk whose loaded
+  // planes z0 - T .. z0 + TZ - 1 + T all lie in the domain, and in a launch
+  // with no phase limit, every iteration k in z0 + T .. kend runs all T stages
+  // (p = k - s lies in [z0 - T + s, z0 + TZ - 1 + T - s] and in the domain),
+  // stores plane k - T (in [z0, z0 + TZ), all inside the domain) and loads
+  // plane k + PD exactly when k + PD <= kload_end. Whole unrolled blocks in
+  // that range run without the per-stage checks.
+  const bool interior = kSteady && g.phase_limit == 0 && z0 - T >= 0 && z0 + TZ - 1 + T < g.n[2];
+  const int k_lo = z0 + T;
+  const int k_fast_end = interior ? kend : kstart - 1;
+  auto block = [&](auto fullc, const int k) __attribute__((always_inline)) {
+    iter(IC<0>{}, fullc, k);
+    iter(IC<1>{}, fullc, k + 1);
+    iter(IC<2>{}, fullc, k + 2);
+    iter(IC<3>{}, fullc, k + 3);
     if constexpr (U > 4) {
-      iter(IC<4>{}, k + 4);
-      iter(IC<5>{}, k + 5);
+      iter(IC<4>{}, fullc, k + 4);
+      iter(IC<5>{}, fullc, k + 5);
     }
-  }
-  if (k <= kend) iter(IC<0>{}, k);
-  if (k + 1 <= kend) iter(IC<1>{}, k + 1);
-  if (k + 2 <= kend) iter(IC<2>{}, k + 2);
+  };
+  int k = kstart;
+  // checked blocks until one fits the steady range, then steady blocks; the
+  // steady range ends at kend, so only the remainder below is checked again
+  for (; k + U - 1 <= kend && !(k >= k_lo && k + U - 1 <= k_fast_end); k += U) block(IC<1>{}, k);
+  if constexpr (kSteady)
+    for (; k + U - 1 <= k_fast_end; k += U) block(IC<0>{}, k);
+  if (k <= kend) iter(IC<0>{}, IC<1>{}, k);
+  if (k + 1 <= kend) iter(IC<1>{}, IC<1>{}, k + 1);
+  if (k + 2 <= kend) iter(IC<2>{}, IC<1>{}, k + 2);
   if constexpr (U > 4) {
-    if (k + 3 <= kend) iter(IC<3>{}, k + 3);
-    if (k + 4 <= kend) iter(IC<4>{}, k + 4);
+    if (k + 3 <= kend) iter(IC<3>{}, IC<1>{}, k + 3);
+    if (k + 4 <= kend) iter(IC<4>{}, IC<1>{}, k + 4);
   }
   flush();
 }

@@ -62,7 +62,7 @@ void candidates(int stencil, int dims, std::vector<Cand>& out) {
     for (int T = 1; T <= 5; ++T)
       for (int fr : {8, 12, 16, 20, 24, 32})
         if (fr > 2 * T && 3 * T * fr <= ot::kW3RegBudget)
-          for (int bz : {8, 16, 20, 32}) out.push_back({T, 64 - 2 * T, fr - 2 * T, bz, 1});
+          for (int bz : {8, 16, 20, 24, 32}) out.push_back({T, 64 - 2 * T, fr - 2 * T, bz, 1});
   }
 }
 

@@ -96,6 +96,11 @@ CASES_3D = [
     (otlib.P3D27, (70, 25, 29), 5, 2, (60, 8, 8), 1),
     (otlib.P3D27, (66, 21, 17), 7, 3, (58, 6, 4), 1),
     (otlib.J3D, (70, 30, 30), 12, 5, (54, 2, 8), 1),
+    # chunks away from both z faces: their middle iterations take the
+    # unchecked steady-state blocks (T, T and a shortened last launch)
+    (otlib.J3D, (70, 20, 60), 7, 3, (58, 6, 16), 1),
+    (otlib.G3D, (64, 20, 50), 5, 2, (60, 8, 12), 1),
+    (otlib.J3D, (62, 12, 50), 3, 1, (62, 6, 16), 1),
 ]
 
 
