@@ -37,6 +37,11 @@ struct KW2 {
 
 constexpr int kWaves = 4;
 
+template <int V>
+struct IC {
+  static constexpr int value = V;
+};
+
 template <int ST, int T, int FR>
 __global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
   constexpr int R = KW2<ST>::R;
@@ -59,115 +64,152 @@ __global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
   const int gx = x0 + lane;
   const bool in_x = gx >= 0 && gx < g.n[0];
   const bool out_x = lane >= H && lane < H + TX;
-  const int64_t base = g.origin + (int64_t)y0 * g.sy + gx;
 
-  uint32_t mxy[FR];
-  bool inside[FR];
+  // Buffer addressing: every field's descriptor starts at footprint row 0
+  // (x = 0), and the byte offset of row i = lane term + row term. A lane or a
+  // row outside the domain has term = range, so its loads return 0 and its
+  // stores are dropped: no branch and no 64-bit address arithmetic per row.
+  // The host keeps FR + 1 rows under 1 GiB, so no offset sum wraps.
+  const uint32_t row_bytes = (uint32_t)(g.sy * 4);
+  const uint32_t range = (uint32_t)(FR + 1) * row_bytes;
+  const int64_t rbase = g.origin + (int64_t)y0 * g.sy;
+  auto rsrc = [&](const float* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p + rbase), (short)0, (int)range, 0x00020000);
+  };
+  const uint32_t lane_ld = in_x ? (uint32_t)gx * 4u : range;
+  const uint32_t lane_st = (in_x && out_x) ? (uint32_t)gx * 4u : range;
+  uint32_t row_ld[FR], mxy[FR];
 #pragma unroll
   for (int i = 0; i < FR; ++i) {
     const int gy = y0 + i;
-    inside[i] = in_x && gy >= 0 && gy < g.n[1];
-    mxy[i] = inside[i] ? 0xFFFFFFFFu : 0u;
+    const bool row_in = gy >= 0 && gy < g.n[1];
+    row_ld[i] = row_in ? (uint32_t)i * row_bytes : range;
+    mxy[i] = (in_x && row_in) ? 0xFFFFFFFFu : 0u;
   }
   if (g.phase_limit == 1) return;
-  auto mask = [&](float v, int i) { return __uint_as_float(__float_as_uint(v) & mxy[i]); };
 
-  float u[NRW][FR];
-  float ro[NRO > 0 ? NRO : 1][FR];
+  // A tile whose whole footprint lies in the domain needs no Dirichlet mask;
+  // each variant holds its own registers from load to store.
+  const bool interior = x0 >= 0 && x0 + 63 < g.n[0] && y0 >= 0 && y0 + FR - 1 < g.n[1];
+  // PHASE: honour the phase-limit split (only the masked variant does, so the
+  // mask-free variant carries no phase checks)
+  auto run = [&](auto maskc, auto phasec) __attribute__((always_inline)) {
+    constexpr bool MASK = decltype(maskc)::value != 0;
+    constexpr bool PHASE = decltype(phasec)::value != 0;
+    auto mask = [&](float v, int i) { return MASK ? __uint_as_float(__float_as_uint(v) & mxy[i]) : v; };
+    float u[NRW][FR];
+    float ro[NRO > 0 ? NRO : 1][FR];
 #pragma unroll
-  for (int k = 0; k < NRW; ++k)
+    for (int k = 0; k < NRW; ++k) {
+      const auto r = rsrc(f.src[k]);
 #pragma unroll
-    for (int i = 0; i < FR; ++i)
-      u[k][i] = (g.phase_limit == 3) ? (float)lane : (inside[i] ? f.src[k][base + (int64_t)i * g.sy] : 0.0f);
+      for (int i = 0; i < FR; ++i)
+        u[k][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(lane_ld + row_ld[i]), 0, 0));
+    }
+    if (PHASE && g.phase_limit == 3) {  // timing split: synthetic input instead of the loads
 #pragma unroll
-  for (int k = 0; k < NRO; ++k)
+      for (int k = 0; k < NRW; ++k)
 #pragma unroll
-    for (int i = 0; i < FR; ++i) ro[k][i] = inside[i] ? f.ro[k][base + (int64_t)i * g.sy] : 0.0f;
+        for (int i = 0; i < FR; ++i) u[k][i] = (float)lane;
+    }
+#pragma unroll
+    for (int k = 0; k < NRO; ++k) {
+      const auto r = rsrc(f.ro[k]);
+#pragma unroll
+      for (int i = 0; i < FR; ++i)
+        ro[k][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(lane_ld + row_ld[i]), 0, 0));
+    }
 
-  // steps s = s_begin..s_end of this launch (phase limits: 2 = first step only,
-  // 3 = all but the first step)
-  const int s_begin = (g.phase_limit == 3) ? 2 : 1;
-  const int s_end = (g.phase_limit == 2) ? 1 : T;
+    // steps s = s_begin..s_end of this launch (phase limits: 2 = first step only,
+    // 3 = all but the first step)
+    const int s_begin = (PHASE && g.phase_limit == 3) ? 2 : 1;
+    const int s_end = (PHASE && g.phase_limit == 2) ? 1 : T;
 
 #pragma unroll
-  for (int s = 1; s <= T; ++s) {
-    if (s > s_end) break;
-    if (s < s_begin) continue;
-    if constexpr (ST == OT_J2D || ST == OT_P2D || ST == OT_G2D || ST == OT_TV2D) {
-      float nu[FR];
+    for (int s = 1; s <= T; ++s) {
+      if constexpr (PHASE) {
+        if (s > s_end) break;
+        if (s < s_begin) continue;
+      }
+      if constexpr (ST == OT_J2D || ST == OT_P2D || ST == OT_G2D || ST == OT_TV2D) {
+        float nu[FR];
 #pragma unroll
-      for (int i = s; i < FR - s; ++i) {
-        const float c = u[0][i], tc = u[0][i - 1], bc = u[0][i + 1];
-        float v;
-        if constexpr (ST == OT_J2D) {
-          v = j2d(from_left(c), c, from_right(c), tc, bc);
-        } else if constexpr (ST == OT_G2D) {
-          v = g2d(from_left(c), c, from_right(c), tc, bc);
-        } else if constexpr (ST == OT_P2D) {
-          v = p2d(from_left(tc), tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
-                  from_right(bc));
-        } else {
-          const float alpha = g.coef[0] != 0.0f ? g.coef[0] : 1.1f;
-          v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc, ro[0][i], ro[1][i],
-                   alpha);
+        for (int i = s; i < FR - s; ++i) {
+          const float c = u[0][i], tc = u[0][i - 1], bc = u[0][i + 1];
+          float v;
+          if constexpr (ST == OT_J2D) {
+            v = j2d(from_left(c), c, from_right(c), tc, bc);
+          } else if constexpr (ST == OT_G2D) {
+            v = g2d(from_left(c), c, from_right(c), tc, bc);
+          } else if constexpr (ST == OT_P2D) {
+            v = p2d(from_left(tc), tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
+                    from_right(bc));
+          } else {
+            const float alpha = g.coef[0] != 0.0f ? g.coef[0] : 1.1f;
+            v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc, ro[0][i], ro[1][i],
+                     alpha);
+          }
+          nu[i] = mask(v, i);
         }
-        nu[i] = mask(v, i);
-      }
 #pragma unroll
-      for (int i = s; i < FR - s; ++i) u[0][i] = nu[i];
-    } else if constexpr (ST == OT_RICIAN2D) {
-      RicianK rk;
-      const float sigma = g.coef[0];
-      rk.sigma2 = sigma != 0.0f ? sigma * sigma : 0.0025f;
-      rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
-      rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
-      // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]
-      float gg[FR];
+        for (int i = s; i < FR - s; ++i) u[0][i] = nu[i];
+      } else if constexpr (ST == OT_RICIAN2D) {
+        RicianK rk;
+        const float sigma = g.coef[0];
+        rk.sigma2 = sigma != 0.0f ? sigma * sigma : 0.0025f;
+        rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
+        rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
+        // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]
+        float gg[FR];
 #pragma unroll
-      for (int i = 2 * s - 1; i <= FR - 2 * s; ++i) {
-        const float c = u[0][i];
-        gg[i] = mask(rician_g(c, from_left(c), from_right(c), u[0][i - 1], u[0][i + 1]), i);
-      }
-      float nu[FR];
+        for (int i = 2 * s - 1; i <= FR - 2 * s; ++i) {
+          const float c = u[0][i];
+          gg[i] = mask(rician_g(c, from_left(c), from_right(c), u[0][i - 1], u[0][i + 1]), i);
+        }
+        float nu[FR];
 #pragma unroll
-      for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) {
-        const float c = u[0][i];
-        const float v = rician_u(c, ro[0][i], from_left(c), from_right(c), u[0][i - 1], u[0][i + 1],
-                                 from_left(gg[i]), from_right(gg[i]), gg[i - 1], gg[i + 1], rk);
-        nu[i] = mask(v, i);
-      }
+        for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) {
+          const float c = u[0][i];
+          const float v = rician_u(c, ro[0][i], from_left(c), from_right(c), u[0][i - 1], u[0][i + 1],
+                                   from_left(gg[i]), from_right(gg[i]), gg[i - 1], gg[i + 1], rk);
+          nu[i] = mask(v, i);
+        }
 #pragma unroll
-      for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) u[0][i] = nu[i];
-    } else {  // FDTD: u[0] = ex, u[1] = ey, u[2] = hz
-      const bool physical = (g.variant & OT_VARIANT_FDTD_PHYSICAL) != 0;
-      // Ey, Ex need Hz at (x-1, y-1): rows [s, FR-s]
+        for (int i = 2 * s; i <= FR - 1 - 2 * s; ++i) u[0][i] = nu[i];
+      } else {  // FDTD: u[0] = ex, u[1] = ey, u[2] = hz
+        const bool physical = (g.variant & OT_VARIANT_FDTD_PHYSICAL) != 0;
+        // Ey, Ex need Hz at (x-1, y-1): rows [s, FR-s]
 #pragma unroll
-      for (int i = s; i <= FR - s; ++i) {
-        const float hz = u[2][i];
-        const float ey = fdtd_ey(u[1][i], hz, u[2][i - 1]);
-        const float ex = fdtd_ex(u[0][i], hz, from_left(hz));
-        u[1][i] = mask(ey, i);
-        u[0][i] = mask(ex, i);
-      }
-      // Hz needs the new Ex, Ey at (x+1, y+1): rows [s, FR-1-s]
+        for (int i = s; i <= FR - s; ++i) {
+          const float hz = u[2][i];
+          const float ey = fdtd_ey(u[1][i], hz, u[2][i - 1]);
+          const float ex = fdtd_ex(u[0][i], hz, from_left(hz));
+          u[1][i] = mask(ey, i);
+          u[0][i] = mask(ex, i);
+        }
+        // Hz needs the new Ex, Ey at (x+1, y+1): rows [s, FR-1-s]
 #pragma unroll
-      for (int i = s; i <= FR - 1 - s; ++i) {
-        const float ex = u[0][i];
-        const float v = fdtd_hz(u[2][i], ex, from_right(ex), u[1][i], u[1][i + 1], physical);
-        u[2][i] = mask(v, i);
+        for (int i = s; i <= FR - 1 - s; ++i) {
+          const float ex = u[0][i];
+          const float v = fdtd_hz(u[2][i], ex, from_right(ex), u[1][i], u[1][i + 1], physical);
+          u[2][i] = mask(v, i);
+        }
       }
     }
-  }
-  if (g.phase_limit != 0) return;
+    if (PHASE && g.phase_limit != 0) return;
 
-  // write back the output tile's interior
+    // write back the output tile's interior (rows H .. H + TY - 1)
 #pragma unroll
-  for (int i = H; i < FR - H; ++i) {
-    if (i < H + TY && out_x && inside[i]) {
+    for (int k = 0; k < NRW; ++k) {
+      const auto r = rsrc(f.dst[k]);
 #pragma unroll
-      for (int k = 0; k < NRW; ++k) f.dst[k][base + (int64_t)i * g.sy] = u[k][i];
+      for (int i = H; i < FR - H; ++i)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, u[k][i]), r,
+                                              (int)(lane_st + (i < H + TY ? row_ld[i] : range)), 0, 0);
     }
-  }
+  };
+  if (interior && g.phase_limit == 0) run(IC<0>{}, IC<0>{});
+  else run(IC<1>{}, IC<1>{});
 }
 
 // Register budget: fields held per lane (rw + ro + Rician's g) x FR.
@@ -176,6 +218,9 @@ constexpr int kFields = KW2<ST>::NRW + KW2<ST>::NRO + (ST == OT_RICIAN2D ? 1 : 0
 
 template <int ST, int T, int FR>
 ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
+  if ((int64_t)(FR + 1) * g.sy * 4 >= (1LL << 30))
+    return set_error(OT_EUNSUPPORTED, "2D row of %lld bytes too large for a %d-row footprint", (long long)(g.sy * 4),
+                     FR);
   if (prepare) return prepare_lds((const void*)&k2w<ST, T, FR>, 0);
   const int nw = g.grid[0] * g.grid[1];
   const int nblk = (nw + kWaves - 1) / kWaves;

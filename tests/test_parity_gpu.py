@@ -240,10 +240,13 @@ FULL_SIZE = [
     (otlib.P3D7, (256, 256, 256), 32, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
     (otlib.FDTD2D, (2048, 2048, 1), 100, (4, (64, 8, 1), 2), ((64, 4, 1), 4)),
     (otlib.J2D, (1024, 1024, 1), 64, (4, (16, 16, 1), 1), ((64, 4, 1), 4)),
+    # the 2D wave kernel at the bench's tiles (mask-free path on interior tiles)
+    (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 18, 1), 1), ((64, 4, 1), 4)),
+    (otlib.J2D, (1024, 1024, 1), 64, (8, (48, 16, 1), 1), ((64, 4, 1), 4)),
 ]
 
 
-@pytest.mark.parametrize("case", FULL_SIZE, ids=[f"{otlib.STENCILS[c[0]]}-full" for c in FULL_SIZE])
+@pytest.mark.parametrize("case", FULL_SIZE, ids=[f"{otlib.STENCILS[c[0]]}-full-T{c[3][0]}" for c in FULL_SIZE])
 def test_full_size_time_tiling_is_bitwise_t1(otlib, case):
     """At the measured workload sizes the time-tiled solve equals the T=1 solve
     bit for bit (size-independent property: same operations in the same order),
