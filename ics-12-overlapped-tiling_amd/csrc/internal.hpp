@@ -59,6 +59,37 @@ struct FieldPtrs {
   const float* ro[2];
 };
 
+// Batched 2D launches (kbatch.hip): one image of the arena, and the arguments
+// of one persistent launch over the (image, tile) work queue.
+struct BImage {
+  int32_t w, h, pitch, reserved;
+  int64_t offset;
+};
+
+struct BArgs {
+  const BImage* images;
+  const int32_t* items;  // 3 ints per item: image, tile x, tile y
+  int32_t* counters;     // one per launch
+  int32_t n_items;
+  int32_t tile[2];
+  int32_t phase_limit, variant;
+  float coef[4];
+  FieldPtrs base;        // arena pointers
+};
+
+// (arguments, t_eff, launch index)
+using BatchKernel = void (*)(BArgs, int, int);
+
+// Wave-per-tile batch kernel advancing t_eff steps with `rows` footprint rows,
+// or nullptr when that instantiation is not compiled (k2w_<stencil>.hip).
+BatchKernel batch_wave_kernel(int stencil, int t_eff, int rows);
+BatchKernel batch_wave_kernel_j2d(int t_eff, int rows);
+BatchKernel batch_wave_kernel_p2d(int t_eff, int rows);
+BatchKernel batch_wave_kernel_g2d(int t_eff, int rows);
+BatchKernel batch_wave_kernel_tv2d(int t_eff, int rows);
+BatchKernel batch_wave_kernel_fdtd2d(int t_eff, int rows);
+BatchKernel batch_wave_kernel_rician2d(int t_eff, int rows);
+
 // Launch one overlapped-tile kernel advancing t_eff steps. Implemented per
 // dimensionality in k1d.hip / k2d.hip / k3d.hip.
 // With prepare=true nothing is launched: the kernel instantiation is looked up

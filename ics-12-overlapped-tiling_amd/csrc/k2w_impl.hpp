@@ -42,20 +42,16 @@ struct IC {
   static constexpr int value = V;
 };
 
+// One wavefront advances output tile (bxi, byi) of the grid described by g by T
+// steps (bxi, byi wave-uniform).
 template <int ST, int T, int FR>
-__global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
+__device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, const int bxi, const int byi) {
   constexpr int R = KW2<ST>::R;
   constexpr int NRW = KW2<ST>::NRW;
   constexpr int NRO = KW2<ST>::NRO;
   constexpr int H = R * T;
   static_assert(FR > 2 * H, "footprint rows must exceed the halo");
   const int lane = threadIdx.x & 63;
-  // wave-uniform by construction; readfirstlane keeps the tile origin in SGPRs
-  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, (int)gridDim.x) * kWaves +
-                                                 (int)(threadIdx.x >> 6));
-  const int nw = g.grid[0] * g.grid[1];
-  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
-  const int bxi = wid % g.grid[0], byi = wid / g.grid[0];
   // tile (TX, TY) with TX + 2H <= 64 lanes and TY + 2H <= FR rows; a plan uses
   // TX = 64 - 2H and FR = TY + 2H, a shortened last launch (smaller t_eff)
   // leaves the spare lanes and rows as dead halo
@@ -212,6 +208,48 @@ __global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
   else run(IC<1>{}, IC<1>{});
 }
 
+template <int ST, int T, int FR>
+__global__ void __launch_bounds__(kWaves * 64) k2w(LaunchGeom g, FieldPtrs f) {
+  // wave-uniform by construction; readfirstlane keeps the tile origin in SGPRs
+  const int wid = __builtin_amdgcn_readfirstlane(dev::xcd_remap((int)blockIdx.x, (int)gridDim.x) * kWaves +
+                                                 (int)(threadIdx.x >> 6));
+  const int nw = g.grid[0] * g.grid[1];
+  if (wid >= nw) return;  // whole wave: no barrier anywhere in this kernel
+  tile<ST, T, FR>(g, f, wid % g.grid[0], wid / g.grid[0]);
+}
+
+// Batched variant: every wavefront pulls (image, tile) items from the launch's
+// queue counter until the queue is empty (each wave leaves once its ticket is
+// past the last item), and advances each item's tile with the same update.
+template <int ST, int T, int FR>
+__global__ void __launch_bounds__(kWaves * 64) k2w_batch(BArgs a, int /*t_eff*/, int launch) {
+  for (;;) {
+    int it = 0;
+    if ((threadIdx.x & 63) == 0) it = atomicAdd(&a.counters[launch], 1);
+    it = __builtin_amdgcn_readfirstlane(it);  // lane 0's ticket, for the whole wave
+    if (it >= a.n_items) break;
+    const int img = a.items[3 * it], tx = a.items[3 * it + 1], ty = a.items[3 * it + 2];
+    const BImage im = a.images[img];
+    LaunchGeom g;
+    g.n[0] = im.w;
+    g.n[1] = im.h;
+    g.n[2] = 1;
+    g.tile[0] = a.tile[0];
+    g.tile[1] = a.tile[1];
+    g.tile[2] = 1;
+    g.grid[0] = g.grid[1] = g.grid[2] = 1;  // unused by the tile update
+    g.sy = im.pitch;
+    g.sz = (int64_t)im.pitch * im.h;
+    g.origin = im.offset;
+    g.phase_limit = a.phase_limit;
+    g.variant = a.variant;
+    for (int i = 0; i < 4; ++i) g.coef[i] = a.coef[i];
+    g.wave_rows = FR;
+    g.reserved = 0;
+    tile<ST, T, FR>(g, a.base, tx, ty);
+  }
+}
+
 // Register budget: fields held per lane (rw + ro + Rician's g) x FR.
 template <int ST>
 constexpr int kFields = KW2<ST>::NRW + KW2<ST>::NRO + (ST == OT_RICIAN2D ? 1 : 0);
@@ -262,6 +300,40 @@ ot_status launch_st(int t_eff, int fr, const LaunchGeom& g, const FieldPtrs& f, 
     case 7: return launch_fr<ST, 7>(fr, g, f, stream, prepare);
     case 8: return launch_fr<ST, 8>(fr, g, f, stream, prepare);
     default: return set_error(OT_EUNSUPPORTED, "2D wave kernels are compiled for T = 1..8 (got %d)", t_eff);
+  }
+}
+
+// Batch kernels, compiled for the same (T, rows) pairs as the grid kernels.
+template <int ST, int T>
+BatchKernel pick_batch_fr(int fr) {
+  constexpr int H = KW2<ST>::R * T;
+  switch (fr) {
+#define FRB(V)                                                                      \
+  case V:                                                                           \
+    if constexpr (V > 2 * H && kFields<ST> * V <= kW2RegBudget) return &k2w_batch<ST, T, V>; \
+    break;
+    FRB(16)
+    FRB(32)
+    FRB(64)
+#undef FRB
+    default:
+      break;
+  }
+  return nullptr;
+}
+
+template <int ST>
+BatchKernel pick_batch(int t_eff, int fr) {
+  switch (t_eff) {
+    case 1: return pick_batch_fr<ST, 1>(fr);
+    case 2: return pick_batch_fr<ST, 2>(fr);
+    case 3: return pick_batch_fr<ST, 3>(fr);
+    case 4: return pick_batch_fr<ST, 4>(fr);
+    case 5: return pick_batch_fr<ST, 5>(fr);
+    case 6: return pick_batch_fr<ST, 6>(fr);
+    case 7: return pick_batch_fr<ST, 7>(fr);
+    case 8: return pick_batch_fr<ST, 8>(fr);
+    default: return nullptr;
   }
 }
 

@@ -5,4 +5,5 @@ namespace ot {
 ot_status launch_2dw_tv2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare) {
   return k2w::launch_st<OT_TV2D>(t_eff, rows, g, f, stream, prepare);
 }
+BatchKernel batch_wave_kernel_tv2d(int t_eff, int rows) { return k2w::pick_batch<OT_TV2D>(t_eff, rows); }
 }  // namespace ot

@@ -5,7 +5,9 @@
 // atomic counter per launch); the workgroups of a launch — as many as fit on
 // the device at once — pull items until the queue is empty, and advance each
 // item's tile by t_eff steps with the same tile update as the per-grid 2D
-// kernels (k2d_impl.hpp). Items are ordered largest image first. The queue
+// kernels: the LDS tile update (k2d_impl.hpp), or, for a tile that fills a
+// wavefront, the wave-per-tile update (k2w_impl.hpp) with every wave pulling
+// its own items. Items are ordered largest image first. The queue
 // counters are cleared by a small kernel at the start of every run (no memset
 // node, so the loop can be captured and replayed as a hipGraph).
 #include <hip/hip_runtime.h>
@@ -22,22 +24,6 @@
 
 namespace ot {
 namespace {
-
-struct BImage {
-  int32_t w, h, pitch, reserved;
-  int64_t offset;
-};
-
-struct BArgs {
-  const BImage* images;
-  const int32_t* items;  // 3 ints per item: image, tile x, tile y
-  int32_t* counters;     // one per launch
-  int32_t n_items;
-  int32_t tile[2];
-  int32_t phase_limit, variant;
-  float coef[4];
-  FieldPtrs base;        // arena pointers
-};
 
 template <int ST, int E>
 __global__ void __launch_bounds__(1024) k2d_batch(BArgs a, int t_eff, int launch) {
@@ -75,7 +61,7 @@ __global__ void zero_counters(int32_t* c, int n) {
   if (i < n) atomicExch(&c[i], 0);
 }
 
-using KernelFn = void (*)(BArgs, int, int);
+using KernelFn = BatchKernel;
 
 template <int ST>
 KernelFn pick_e(int ept) {
@@ -98,6 +84,22 @@ KernelFn pick(int stencil, int ept) {
     default: return nullptr;
   }
 }
+
+}  // namespace
+
+BatchKernel batch_wave_kernel(int stencil, int t_eff, int rows) {
+  switch (stencil) {
+    case OT_J2D: return batch_wave_kernel_j2d(t_eff, rows);
+    case OT_P2D: return batch_wave_kernel_p2d(t_eff, rows);
+    case OT_G2D: return batch_wave_kernel_g2d(t_eff, rows);
+    case OT_TV2D: return batch_wave_kernel_tv2d(t_eff, rows);
+    case OT_FDTD2D: return batch_wave_kernel_fdtd2d(t_eff, rows);
+    case OT_RICIAN2D: return batch_wave_kernel_rician2d(t_eff, rows);
+    default: return nullptr;
+  }
+}
+
+namespace {
 
 #define OT_HIP(call)                                                                      \
   do {                                                                                    \
@@ -132,13 +134,32 @@ struct ot_batch {
   bool per_image_mode = false;  // set for the duration of one ot_batch_run
 };
 
-static ot_status make_shape(int stencil, int t_eff, int ept, const int32_t tile[3], LaunchShape& s, int n_items) {
-  s.fn = pick(stencil, ept);
-  if (!s.fn) return set_error(OT_EUNSUPPORTED, "batched kernels are compiled for 2D stencils with 1, 2 or 4 "
-                                               "elements per thread");
-  shape_2d(stencil, t_eff, ept, tile, &s.bx, &s.by, &s.lds);
-  if (s.bx * s.by > kMaxThreads) return set_error(OT_ERESOURCE, "batch tile needs %d threads", s.bx * s.by);
-  if (s.lds > kMaxLdsBytes) return set_error(OT_ERESOURCE, "batch tile needs %d B of LDS", s.lds);
+// Launch shape of one persistent batch launch advancing t_eff steps with the
+// batch's tile (time tile T). A tile that fills a wavefront (x tile + 2rT = 64
+// with a compiled footprint row count, as the per-grid wave kernel takes) runs
+// the wave-per-tile batch kernel, four independent waves per workgroup; other
+// tiles run the LDS tile update.
+static ot_status make_shape(int stencil, int T, int t_eff, int ept, const int32_t tile[3], LaunchShape& s,
+                            int n_items) {
+  const int rows = k2w_rows(stencil, T, tile);
+  int items_per_wg = 1;
+  if (rows > 0) {
+    s.fn = batch_wave_kernel(stencil, t_eff, rows);
+    if (!s.fn)
+      return set_error(OT_EUNSUPPORTED, "batched wave kernel not compiled for T=%d with %d footprint rows", t_eff,
+                       rows);
+    s.bx = 256;
+    s.by = 1;
+    s.lds = 0;
+    items_per_wg = 4;
+  } else {
+    s.fn = pick(stencil, ept);
+    if (!s.fn) return set_error(OT_EUNSUPPORTED, "batched kernels are compiled for 2D stencils with 1, 2 or 4 "
+                                                 "elements per thread");
+    shape_2d(stencil, t_eff, ept, tile, &s.bx, &s.by, &s.lds);
+    if (s.bx * s.by > kMaxThreads) return set_error(OT_ERESOURCE, "batch tile needs %d threads", s.bx * s.by);
+    if (s.lds > kMaxLdsBytes) return set_error(OT_ERESOURCE, "batch tile needs %d B of LDS", s.lds);
+  }
   const ot_status pst = prepare_lds((const void*)s.fn, s.lds);
   if (pst != OT_OK) return pst;
   int per_cu = 0;
@@ -147,7 +168,7 @@ static ot_status make_shape(int stencil, int t_eff, int ept, const int32_t tile[
   int dev = 0, cus = 0;
   OT_HIP(hipGetDevice(&dev));
   OT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  s.wgs = std::min(n_items, per_cu * cus);
+  s.wgs = std::min((n_items + items_per_wg - 1) / items_per_wg, per_cu * cus);
   return OT_OK;
 }
 
@@ -173,6 +194,7 @@ static ot_status per_image_enqueue(ot_batch* b, float* const* rw_a, float* const
       g.sy = im.pitch;
       g.sz = (int64_t)im.pitch * im.height;
       g.origin = im.offset;
+      g.wave_rows = k2w_rows(b->p.stencil, T, g.tile);  // the kernel family of the batched launch
       g.phase_limit = b->p.phase_limit;
       g.variant = b->p.variant;
       for (int c = 0; c < 4; ++c) g.coef[c] = b->p.coef[c];
@@ -285,15 +307,17 @@ ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_im
   b->num_rw = kTraits[p.stencil].num_rw;
   b->num_ro = kTraits[p.stencil].num_ro;
   b->host_images.assign(images, images + n_images);
-  ot_status st = make_shape(p.stencil, p.time_tile, p.elems_per_thread, tile, b->full, b->n_items);
+  ot_status st = make_shape(p.stencil, p.time_tile, p.time_tile, p.elems_per_thread, tile, b->full, b->n_items);
   const int last = p.time_steps % p.time_tile;
-  if (st == OT_OK) st = last ? make_shape(p.stencil, last, p.elems_per_thread, tile, b->last, b->n_items) : OT_OK;
+  if (st == OT_OK)
+    st = last ? make_shape(p.stencil, p.time_tile, last, p.elems_per_thread, tile, b->last, b->n_items) : OT_OK;
   if (st == OT_OK && last == 0) b->last = b->full;
   // the per-image baseline launches the per-grid kernels of the same tile
   if (st == OT_OK) {
     LaunchGeom g;
     std::memset(&g, 0, sizeof(g));
     for (int a = 0; a < 3; ++a) g.tile[a] = tile[a];
+    g.wave_rows = k2w_rows(p.stencil, p.time_tile, tile);
     FieldPtrs f;
     std::memset(&f, 0, sizeof(f));
     st = launch_2d(p.stencil, p.time_tile, p.elems_per_thread, g, f, nullptr, true);

@@ -90,3 +90,70 @@ def run_batch_host(params: Params, images, rw_arenas, ro_arenas=(), use_graph=Fa
         buf.free()
     bt.close()
     return out, tm
+
+
+def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 77):
+    """Time every candidate tile of a denoising batch both ways: the batched
+    work-queue launch (hipGraph replay, best of 3) and one per-grid launch per
+    image and time tile (faster of eager and graph). Candidates are LDS tiles
+    (T 1..6, 9 block shapes, 1/2/4 elements per thread) and the tiles that fill
+    a wavefront (x tile + 2rT = 64, footprint rows 16 or 32). Returns one dict
+    per configuration that builds."""
+    import itertools
+    import time
+
+    st = otlib.stencil_id(stencil)
+    R = otlib.stencil_info(st).radius
+    w, h = draw_sizes(n_images, seed=seed)
+    ims, total = layout(w, h)
+    f = phantom_arena(ims, total, seed=seed, sigma=0.05)
+    coef = [0.05, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
+    ro = [f] if st == otlib.RICIAN2D else [np.ones(total, np.float32), f]
+    A, B = DeviceBuffer.from_array(f), DeviceBuffer.from_array(f)
+    RO = [DeviceBuffer.from_array(x) for x in ro]
+
+    def best_ms(bt, graph, per_image, reps):
+        bt.run([A], [B], RO, use_graph=graph, per_image=per_image)
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            bt.run([A], [B], RO, use_graph=graph, per_image=per_image)
+            best = min(best, time.perf_counter() - t0)
+        return best * 1e3
+
+    cfgs = [(T, bx, by, e) for T, (bx, by), e in itertools.product(
+        range(1, 7), [(32, 8), (32, 16), (64, 8), (64, 16), (32, 32), (48, 32), (64, 32), (128, 8), (16, 16)],
+        (1, 2, 4)) if by % e == 0]
+    for T in range(1, 9):
+        x = 64 - 2 * R * T
+        for fr in (16, 32):
+            if x > 0 and fr > 2 * R * T:
+                cfgs.append((T, x, fr - 2 * R * T, 1))
+    out = []
+    try:
+        for T, bx, rows, e in cfgs:
+            try:
+                bt = Batch(otlib.make_params(st, steps=iterations, T=T, x=bx, y=rows // e, e=e, coef=coef), ims)
+            except otlib.OtError:
+                continue
+            try:
+                out.append(dict(T=T, tile=[bx, rows], e=e, wave=int(bx + 2 * R * T == 64),
+                                batched_ms=round(best_ms(bt, True, False, 3), 4),
+                                per_image_ms=round(min(best_ms(bt, False, True, 1), best_ms(bt, True, True, 1)), 4)))
+            finally:
+                bt.close()
+    finally:
+        for buf in [A, B] + RO:
+            buf.free()
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    for name in (sys.argv[1:] or ["rician-denoise-2d", "cdsc-tv-update-2d"]):
+        res = sweep(name)
+        for key in ("batched_ms", "per_image_ms"):
+            top = sorted(res, key=lambda r: r[key])[:5]
+            print(json.dumps({"stencil": name, "configs": len(res), "fastest_by": key, "top": top}), flush=True)

@@ -84,6 +84,9 @@ def _embed(d, interior):
 @pytest.mark.parametrize("st,T,bx,by,e", [
     (otlib.RICIAN2D, 3, 16, 8, 2), (otlib.RICIAN2D, 1, 32, 16, 1), (otlib.TV2D, 4, 32, 4, 4),
     (otlib.J2D, 5, 16, 16, 1), (otlib.FDTD2D, 2, 16, 8, 2), (otlib.G2D, 3, 32, 8, 1), (otlib.P2D, 2, 16, 8, 2),
+    # tiles that fill a wavefront (x tile + 2rT = 64): the wave-per-tile batch kernel
+    (otlib.RICIAN2D, 2, 56, 24, 1), (otlib.TV2D, 3, 58, 26, 1), (otlib.J2D, 4, 56, 8, 1),
+    (otlib.FDTD2D, 2, 60, 12, 1), (otlib.G2D, 5, 54, 22, 1), (otlib.P2D, 8, 48, 48, 1),
 ])
 def test_batch_equals_per_image_and_oracle(otlib, st, T, bx, by, e):
     coef = (0.1, 0, 0, 0) if st == otlib.TV2D else (0, 0, 0, 0)
