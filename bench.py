@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # once, every ro field read once).
 WORKLOADS = {
     "jacobi3d": dict(stencil="jacobi-3d", n=(256, 256, 256), steps=32,
-                     tiled=dict(T=3, x=58, y=6, z=20, e=1), t1=dict(T=1, x=62, y=6, z=8, e=1),
+                     tiled=dict(T=3, x=58, y=6, z=19, e=1), t1=dict(T=1, x=62, y=6, z=8, e=1),
                      bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
     "jacobi2d": dict(stencil="jacobi-2d", n=(1024, 1024, 1), steps=64,
                      tiled=dict(T=8, x=48, y=16, z=1, e=1), t1=dict(T=1, x=16, y=16, z=1, e=4),
@@ -51,10 +51,10 @@ WORKLOADS = {
                    tiled=dict(T=7, x=50, y=18, z=1, e=1), t1=dict(T=1, x=62, y=14, z=1, e=1),
                    bytes_pp=24, data="Ex = Ey = 0, Hz Gaussian pulse (amplitude 1, sigma 8) at the centre"),
     "gradient3d": dict(stencil="gradient-3d", n=(256, 256, 256), steps=64,
-                       tiled=dict(T=2, x=64, y=8, z=32, e=2), t1=dict(T=1, x=32, y=8, z=16, e=4),
+                       tiled=dict(T=2, x=64, y=8, z=32, e=2), t1=dict(T=1, x=62, y=6, z=14, e=1),
                        bytes_pp=8, data="sin(2 pi x/256) sin(2 pi y/256) sin(2 pi z/256) + U(-1e-3, 1e-3), seed 7"),
     "poisson3d-rhs": dict(stencil="poisson-3d-rhs", n=(256, 256, 256), steps=32,
-                          tiled=dict(T=2, x=64, y=8, z=32, e=2), t1=dict(T=1, x=64, y=8, z=32, e=4),
+                          tiled=dict(T=2, x=60, y=4, z=29, e=1), t1=dict(T=1, x=62, y=6, z=64, e=1),
                           bytes_pp=12, data="u0 = 0, f uniform [-1, 1) from seed 11"),
 }
 # batch workloads: (T, block x, block y rows, elements per thread) of the

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   constexpr int NQ = KW<ST>::NQ;
   constexpr int U = KW<ST>::U;
   constexpr int PD = KW<ST>::PD;
-  constexpr bool kSteady = L == 1 && !RHS;  // see the steady state below
+  constexpr bool kSteady = L == 1;  // see the steady state below
   static_assert(U % PD == 0 && U % NQ == 0, "unroll must cycle the prefetch ring and the queue");
   static_assert(FR > 2 * T, "footprint rows must exceed the halo");
   const int lane = threadIdx.x & 63;
@@ -260,8 +260,8 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
     if constexpr (!FULL) __builtin_amdgcn_sched_barrier(0);
   };
 
-  // Steady state (7-point stencils without a right-hand side; the others keep
-  // one checked loop, which holds fewer registers). In a chunk whose loaded
+  // Steady state (7-point stencils; the 27-point one keeps one checked loop,
+  // which holds fewer registers). In a chunk whose loaded
   // planes z0 - T .. z0 + TZ - 1 + T all lie in the domain, and in a launch
   // with no phase limit, every iteration k in z0 + T .. kend runs all T stages
   // (p = k - s lies in [z0 - T + s, z0 + TZ - 1 + T - s] and in the domain),

@@ -32,7 +32,8 @@ struct Cand {
 // Sweep ranges per dimensionality. They cover every instantiation compiled in
 // k1d.hip (ept 1..8), k2d.hip (ept 1..12), k3d_impl.hpp (T 1..8, ept 1/2/4, 8
 // for T <= 2) and k3w_impl.hpp (T 1..5, footprint rows 8..32).
-void candidates(int stencil, int dims, std::vector<Cand>& out) {
+void candidates(const ot_params& base, int dims, std::vector<Cand>& out) {
+  const int stencil = base.stencil;
   out.clear();
   if (dims == 1) {
     for (int T = 1; T <= 8; ++T)
@@ -58,11 +59,29 @@ void candidates(int stencil, int dims, std::vector<Cand>& out) {
           for (int bz : {8, 16, 32})
             for (int e : {1, 2, 4, 8})
               if (e < 8 || T <= 2) out.push_back({T, bx, by, bz, e});
-    // wave-pipelined kernel: x tile + 2T fills the 64 lanes, footprint rows FR
+    // wave-pipelined kernel: x tile + 2T fills the 64 lanes, footprint rows FR.
+    // Besides fixed z-chunks, the chunks that make the wave count just fill k
+    // waves per SIMD (k = 1..6, 4 SIMDs per CU of the default machine), so that
+    // no SIMD runs one wave more than most others.
+    ot_arch arch;
+    ot_arch_default(&arch);
+    const int64_t simds = 4 * (int64_t)arch.num_cu;
+    const int nx = base.nx > 0 ? base.nx : base.problem_size, ny = base.ny > 0 ? base.ny : base.problem_size;
+    const int nz = base.nz > 0 ? base.nz : base.problem_size;
     for (int T = 1; T <= 5; ++T)
       for (int fr : {8, 12, 16, 20, 24, 32})
-        if (fr > 2 * T && 3 * T * fr <= ot::kW3RegBudget)
-          for (int bz : {8, 16, 20, 24, 32}) out.push_back({T, 64 - 2 * T, fr - 2 * T, bz, 1});
+        if (fr > 2 * T && 3 * T * fr <= ot::kW3RegBudget) {
+          const int tx = 64 - 2 * T, ty = fr - 2 * T;
+          std::vector<int> bzs = {8, 16, 20, 24, 32};
+          const int64_t tiles_xy = (int64_t)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
+          for (int k = 1; k <= 6 && nx > 0 && ny > 0 && nz > 0; ++k) {
+            const int64_t chunks = k * simds / tiles_xy;
+            if (chunks < 1) continue;
+            const int bz = (int)((nz + chunks - 1) / chunks);
+            if (bz >= 4 && std::find(bzs.begin(), bzs.end(), bz) == bzs.end()) bzs.push_back(bz);
+          }
+          for (int bz : bzs) out.push_back({T, tx, ty, bz, 1});
+        }
   }
 }
 
@@ -113,7 +132,7 @@ ot_status ot_tune_space(const ot_params* base, ot_params* out, int32_t capacity,
   if (base->stencil < 0 || base->stencil >= OT_NUM_STENCILS)
     return ot::set_error(OT_EINVAL, "ot_tune_space: unknown stencil %d", base->stencil);
   std::vector<Cand> cands;
-  candidates(base->stencil, kTraits[base->stencil].dims, cands);
+  candidates(*base, kTraits[base->stencil].dims, cands);
   int n = 0;
   for (const Cand& c : cands) {
     ot_params p = with_cand(*base, c);

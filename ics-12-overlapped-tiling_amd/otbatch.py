@@ -137,7 +137,9 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
             except otlib.OtError:
                 continue
             try:
-                out.append(dict(T=T, tile=[bx, rows], e=e, wave=int(bx + 2 * R * T == 64),
+                fr = rows + 2 * R * T  # footprint rows of a wave tile
+                wave = bx + 2 * R * T == 64 and e == 1 and fr in (16, 32, 64) and 3 * fr <= 128
+                out.append(dict(T=T, tile=[bx, rows], e=e, wave=int(wave),
                                 batched_ms=round(best_ms(bt, True, False, 3), 4),
                                 per_image_ms=round(min(best_ms(bt, False, True, 1), best_ms(bt, True, True, 1)), 4)))
             finally:

@@ -235,9 +235,9 @@ def test_enqueue_only_solves_match_synchronous(otlib):
 
 FULL_SIZE = [
     # (stencil, extents, steps, tiled (T, block, ept), T=1 (block, ept)): the bench workloads
-    (otlib.J3D, (256, 256, 256), 32, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
+    (otlib.J3D, (256, 256, 256), 32, (3, (58, 6, 19), 1), ((64, 4, 32), 4)),
     (otlib.G3D, (256, 256, 256), 16, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
-    (otlib.P3D7, (256, 256, 256), 32, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
+    (otlib.P3D7, (256, 256, 256), 32, (2, (60, 4, 29), 1), ((64, 4, 32), 4)),
     (otlib.FDTD2D, (2048, 2048, 1), 100, (4, (64, 8, 1), 2), ((64, 4, 1), 4)),
     (otlib.J2D, (1024, 1024, 1), 64, (4, (16, 16, 1), 1), ((64, 4, 1), 4)),
     # the 2D wave kernel at the bench's tiles (mask-free path on interior tiles)

@@ -37,6 +37,19 @@ def test_space_holds_both_3d_kernel_families(otlib):
     assert 0 < len(wave) < len(sp)
 
 
+def test_space_balances_wave_counts_over_z(otlib):
+    """3D wave tiles also get the z-chunks that make the wave count just fill k
+    waves per SIMD (k = 1..6) on 256 CUs x 4 SIMDs: J3D 256^3 with a 58x6 tile has
+    5 x 43 = 215 waves per z-layer of chunks, so k = 3 gives 3 * 1024 // 215 = 14
+    chunks of ceil(256 / 14) = 19 planes, k = 2 gives 9 chunks of 29."""
+    sp = ottune.space(_p("jacobi-3d", (256, 256, 256), 32, 1, 16, 4, 4, 1))
+    bz = {q.block_z for q in sp if (q.time_tile, q.block_x, q.block_y) == (3, 58, 6)}
+    assert {8, 16, 20, 24, 32} <= bz and {19, 29, 64} <= bz
+    for k in range(1, 7):
+        chunks = k * 1024 // 215
+        assert -(-256 // chunks) in bz
+
+
 def test_features(otlib):
     p = _p("jacobi-3d", (256, 128, 64), 32, 3, 58, 6, 20, 1)
     f = ottune.features(p)
