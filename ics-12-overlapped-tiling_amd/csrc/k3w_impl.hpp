@@ -110,6 +110,11 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
     mxy[i] = (in_x && row_in) ? 0xFFFFFFFFu : 0u;
   }
   if (g.phase_limit == 1) return;
+  // lane + row term of every footprint row, for the steady-state loads
+  constexpr bool kSoff = kSteady && T >= 2;
+  uint32_t vld[kSoff ? FR : 1];
+#pragma unroll
+  for (int i = 0; i < (kSoff ? FR : 1); ++i) vld[i] = lane_ld + row_ld[i];
 
   float q[T][NQ][FR];
   float fq[T + 1][FR];
@@ -133,11 +138,26 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   const int s_end = (g.phase_limit == 2) ? 1 : T;
 
   float pf[PD][FR], pfr[PD][FR];
-  // FULL = 0: a steady-state iteration of an interior chunk (see below), whose
-  // plane k is loaded exactly when k <= kload_end
+  // FULL = 0: a steady-state iteration of an interior chunk (see below). No
+  // plane it fetches lies outside the chunk's domain planes, and a plane past
+  // kload_end is never read by a steady or later iteration, so its fetch may
+  // read plane kload_end instead. The plane offset then goes in the scalar
+  // offset, and the vector offset is the lane + row term computed once per
+  // wave: no vector instruction per load. A lane or row outside the domain has
+  // term >= range, which is out of range with or without the scalar offset.
   auto fetch = [&](int k, auto slotc, auto fullc) __attribute__((always_inline)) {
     constexpr int SL = decltype(slotc)::value;
     constexpr bool FULL = decltype(fullc)::value != 0;
+    if constexpr (!FULL && kSoff) {
+      const int soff = (int)((uint32_t)(min(k, kload_end) - (z0 - T)) * plane_bytes);
+#pragma unroll
+      for (int i = 0; i < FR; ++i) {
+        pf[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)vld[i], soff, 0));
+        if constexpr (RHS)
+          pfr[SL][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rro, (int)vld[i], soff, 0));
+      }
+      return;
+    }
     const bool live = FULL ? (do_loads && k >= 0 && k < g.n[2] && k <= kload_end) : (k <= kload_end);
     // plane term = range for a plane that is not loaded: every offset below
     // then reaches the range (the three terms stay under 3 GiB, no wrap)
