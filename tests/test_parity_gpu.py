@@ -263,3 +263,34 @@ def test_full_size_time_tiling_is_bitwise_t1(otlib, case):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     one = _params(st, n3, 1, T, blk, ept)
     check_against_oracle(one, rw, ro)
+
+
+PHASE_CASES = [
+    # one case per kernel family (1D LDS, 2D LDS, 2D wave-per-tile, 2D row-streaming,
+    # 3D LDS, 3D wave-pipelined with and without its steady-state blocks)
+    (otlib.J1D3, (3000, 1, 1), 6, 2, (256, 1, 1), 2),
+    (otlib.J2D, (130, 77, 1), 6, 2, (16, 8, 1), 2),
+    (otlib.FDTD2D, (130, 77, 1), 6, 2, (60, 12, 1), 1),
+    (otlib.J2D, (200, 150, 1), 6, 2, (60, 64, 1), 1),
+    (otlib.J3D, (40, 37, 35), 6, 2, (16, 8, 8), 2),
+    (otlib.J3D, (70, 20, 60), 6, 3, (58, 6, 16), 1),
+    (otlib.P3D27, (70, 25, 29), 6, 2, (60, 8, 8), 1),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", PHASE_CASES, ids=_ids(PHASE_CASES))
+@pytest.mark.parametrize("phase", [1, 2, 3])
+def test_phase_limited_launches_store_nothing(otlib, case, phase):
+    """The timing split of the driver's -p option (1: set-up only, 2: loads and the
+    first step, 3: every step but the first, from a synthetic tile) runs each
+    kernel family part way and stores no result: every field is left bit for bit
+    as it was, and the run completes without an error."""
+    st, n3, steps, T, blk, ept = case
+    p = _params(st, n3, steps, T, blk, ept, p=phase)
+    d = otlib.derive(p)
+    rw, ro = random_inputs(st, domain_shape(d), seed=5)
+    print(f"phase {phase} {otlib.STENCILS[st]} {n3} T={T} {blk} e{ept}", flush=True)
+    out, _ = run_gpu(p, rw, ro)
+    for a, b in zip(out, rw):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
