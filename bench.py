@@ -180,7 +180,14 @@ def cpu_baseline(w, n3, steps):
     d = otlib.derive(plan_params(w, dict(T=1, x=16, y=4, z=4 if n3[2] > 1 else 1, e=1), n3, steps))
     rw, ro = make_inputs(w, d)
     sl = otlib.interior_slices(d)
-    _, dt = oracle.run(st, [x[sl].copy() for x in rw], [x[sl].copy() for x in ro], steps=steps, timed=True)
+    # single-threaded, pinned to one core of the host for the duration of the run
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
+    try:
+        _, dt = oracle.run(st, [x[sl].copy() for x in rw], [x[sl].copy() for x in ro], steps=steps, timed=True)
+    finally:
+        os.sched_setaffinity(0, allowed)
     pts = float(n3[0]) * n3[1] * n3[2]
     shape = "x".join(str(v) for v in n3 if v > 1)
     return {
@@ -189,9 +196,20 @@ def cpu_baseline(w, n3, steps):
         "cores": 1,
         "kind": "port",
         "sample": f"whole workload: {w['stencil']} {shape}, {steps} time steps, oracle C restatement of the "
-                  f"reference CPU loop, gcc -O2 -ffp-contract=off, 1 thread of "
-                  f"{os.cpu_count()} ({platform.processor() or platform.machine()}), {dt:.2f} s",
+                  f"reference CPU loop, gcc -O2 -ffp-contract=off, 1 thread pinned to core {core} of "
+                  f"{len(allowed)} available ({_cpu_model()}), {dt:.2f} s",
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def bench_batch(args, stencil_name):
@@ -386,6 +404,9 @@ def main():
                      "kernel": kernel_name(w, d), "avg_launch_us": round(launch_s * 1e6, 2),
                      "eager_avg_launch_us": round(eager_launch_s * 1e6, 2),
                      "bytes_per_launch": bytes_per_launch},
+        # the reference drivers' "Actual GFlop/s": useful points x flops per
+        # point x steps / time (jacobi-2d.cpp:903-1013)
+        "actual_gflops": round(gst * otlib.stencil_info(otlib.stencil_id(w["stencil"])).flops_per_point, 1),
     }
     if not args.no_t1:
         t1 = w["t1"]
