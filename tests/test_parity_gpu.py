@@ -101,6 +101,12 @@ CASES_3D = [
     (otlib.J3D, (70, 20, 60), 7, 3, (58, 6, 16), 1),
     (otlib.G3D, (64, 20, 50), 5, 2, (60, 8, 12), 1),
     (otlib.J3D, (62, 12, 50), 3, 1, (62, 6, 16), 1),
+    # tiles whose lanes and rows all lie inside the domain in x and y (the tile
+    # body without the per-point mask), in checked and steady-state iterations
+    (otlib.J3D, (200, 40, 60), 7, 3, (58, 6, 16), 1),
+    (otlib.J3D, (190, 44, 40), 5, 1, (62, 6, 8), 1),
+    (otlib.G3D, (200, 40, 50), 5, 2, (60, 8, 12), 1),
+    (otlib.P3D7, (200, 40, 60), 7, 2, (60, 4, 16), 1),
 ]
 
 
