@@ -56,6 +56,11 @@ WORKLOADS = {
     "poisson3d-rhs": dict(stencil="poisson-3d-rhs", n=(256, 256, 256), steps=32,
                           tiled=dict(T=2, x=60, y=4, z=29, e=1), t1=dict(T=1, x=62, y=6, z=64, e=1),
                           bytes_pp=12, data="u0 = 0, f uniform [-1, 1) from seed 11"),
+    # compute-bound (4 rsqrt and 5 divisions per point): the sweep's best tile is
+    # a T=1 tile, so the time-tiled and T=1 lines coincide
+    "cs3d": dict(stencil="cs-3d", n=(256, 256, 256), steps=64,
+                 tiled=dict(T=1, x=62, y=6, z=10, e=1), t1=dict(T=1, x=62, y=6, z=10, e=1),
+                 bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
 }
 # batch workloads: (T, block x, block y rows, elements per thread) of the
 # batched work-queue launch, and of the per-image baseline (one launch per image
@@ -104,7 +109,7 @@ def calibrate_copy(mib):
 def make_inputs(w, d):
     """Host arrays (layout of derived d) for the rw and ro fields of workload w."""
     st = otlib.stencil_id(w["stencil"])
-    if st in (otlib.J3D, otlib.J2D):
+    if st in (otlib.J3D, otlib.J2D, otlib.CS3D):
         return [otlib.fill_reference_uniform(d, 123456)], []
     if st == otlib.FDTD2D:
         z = otlib.empty_field(d)
@@ -177,7 +182,8 @@ def cpu_baseline(w, n3, steps):
     import oracle
 
     st = otlib.stencil_id(w["stencil"])
-    d = otlib.derive(plan_params(w, dict(T=1, x=16, y=4, z=4 if n3[2] > 1 else 1, e=1), n3, steps))
+    # the layout of the workload's T=1 plan (a tile every stencil accepts)
+    d = otlib.derive(plan_params(w, w["t1"], n3, steps))
     rw, ro = make_inputs(w, d)
     sl = otlib.interior_slices(d)
     # single-threaded, pinned to one core of the host for the duration of the run
@@ -283,7 +289,7 @@ def tuning_leg(w, n3, steps, headline):
     top 3) and the analytic model's pick, all on this GPU now."""
     import ottune
 
-    base = plan_params(w, dict(T=1, x=16, y=4, z=4 if n3[2] > 1 else 1, e=1), n3, steps)
+    base = plan_params(w, w["t1"], n3, steps)
     space = ottune.space(base)
     d = otlib.derive(base)
     rw, ro = make_inputs(w, d)

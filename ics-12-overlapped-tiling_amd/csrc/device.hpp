@@ -105,6 +105,25 @@ __device__ __forceinline__ float p3d7(float xm, float xp, float ym, float yp, fl
                                       float f) {
   return ((((((xm + xp) + ym) + yp) + zm) + zp) - f) / 6.0f;
 }
+// cs-3d.cpp:286-308 (kernel constants :161-163; eps taken as the float 1e-9f,
+// pow(d, 2.0f) as d*d). Arguments by offset (dz, dy, dx) from the centre u0;
+// each tap is the address of the same-named value in the on-chip steps
+// (cs-3d.cpp:325-379), where val_0_m1_0 is row y-1 and val_0_p1_0 row y+1 (the
+// first step of the reference's time tile reads those two the other way round,
+// cs-3d.cpp:258-263; not reproduced). c1..c4 weight the centre's forward
+// neighbours and its z-1, y-1, x-1 neighbours by the centre value over the
+// forward-difference gradient magnitude at the centre and at those neighbours.
+__device__ __forceinline__ float cs3d(float u0, float zm, float zp, float ym, float yp, float xm, float xp,
+                                      float zm_yp, float zm_xp, float zp_ym, float ym_xp, float zp_xm,
+                                      float yp_xm) {
+  const float eps = 1e-9f, sum_aref = 1.1f, alpha = 1.1f;
+  const float c1 = (u0 * rsq(((eps + sq(zp - u0)) + sq(yp - u0)) + sq(xp - u0))) / sum_aref;
+  const float c2 = (u0 * rsq(((eps + sq(u0 - zm)) + sq(zm_yp - zm)) + sq(zm_xp - zm))) / sum_aref;
+  const float c3 = (u0 * rsq(((eps + sq(zp_ym - ym)) + sq(u0 - ym)) + sq(ym_xp - ym))) / sum_aref;
+  const float c4 = (u0 * rsq(((eps + sq(zp_xm - xm)) + sq(yp_xm - xm)) + sq(u0 - xm))) / sum_aref;
+  return ((((alpha * u0 + c1 * ((zp + yp) + xp)) + c2 * zm) + c3 * ym) + c4 * xm) /
+         ((((alpha + 3.0f * c1) + c2) + c3) + c4);
+}
 
 // cdsc-tv-update-2d.cpp:290-309. c3 duplicates c1 as in the reference (its
 // MATLAB comment repeats c1's formula); V and F read-only, alpha a parameter.

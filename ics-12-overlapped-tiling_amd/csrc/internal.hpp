@@ -121,6 +121,7 @@ ot_status launch_2dw_tv2d(int t_eff, int rows, const LaunchGeom& g, const FieldP
 ot_status launch_2dw_fdtd2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_2dw_rician2d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_g3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
+ot_status launch_3dw_cs3d(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_p3d7(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 ot_status launch_3dw_p3d27(int t_eff, int rows, const LaunchGeom& g, const FieldPtrs& f, void* stream, bool prepare);
 

@@ -1,8 +1,8 @@
 // k3w_impl.hpp — wave-pipelined overlapped time-tiled 3D kernels.
 //
 // A second kernel family for the 3D stencils (jacobi-3d.cpp:120-432,
-// poisson-3d.cpp:120-398, gradient-3d.cpp:120-434) with no LDS and no
-// barriers: every wavefront is an independent tile.
+// poisson-3d.cpp:120-398, gradient-3d.cpp:120-434, cs-3d.cpp:156-430) with no
+// LDS and no barriers: every wavefront is an independent tile.
 //   * The 64 lanes of a wavefront are 64 consecutive x columns of the
 //     footprint; every lane holds a whole column strip of FR footprint rows in
 //     registers. x-neighbours come from the adjacent lanes through DPP
@@ -244,6 +244,11 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
           const float xm = from_left(c), xp = from_right(c);
           if constexpr (ST == OT_J3D) v = j3d(xm, xp, ym, yp, zm, zp, c);
           else if constexpr (ST == OT_G3D) v = g3d(xm, xp, ym, yp, zm, zp, c);
+          else if constexpr (ST == OT_CS3D)
+            // diagonal taps: (z-1, y+1), (z-1, x+1), (z+1, y-1), (y-1, x+1),
+            // (z+1, x-1), (y+1, x-1)
+            v = cs3d(c, zm, zp, ym, yp, xm, xp, q[s - 1][Q2][i + 1], from_right(zm), q[s - 1][QN][i - 1],
+                     from_right(ym), from_left(zp), from_left(yp));
           else v = p3d7(xm, xp, ym, yp, zm, zp, fq[s][i]);
         } else {
           // poisson-3d.cpp:282-293: planes p-1, p, p+1 of level s-1 were

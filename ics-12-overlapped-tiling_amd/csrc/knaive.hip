@@ -59,6 +59,9 @@ __global__ void __launch_bounds__(256) naive_step(NaiveGeom g, const float* __re
     v = g3d(U(-1, 0, 0), U(1, 0, 0), U(0, -1, 0), U(0, 1, 0), U(0, 0, -1), U(0, 0, 1), U(0, 0, 0));
   else if constexpr (ST == OT_P3D7)
     v = p3d7(U(-1, 0, 0), U(1, 0, 0), U(0, -1, 0), U(0, 1, 0), U(0, 0, -1), U(0, 0, 1), r0[i]);
+  else if constexpr (ST == OT_CS3D)
+    v = cs3d(U(0, 0, 0), U(0, 0, -1), U(0, 0, 1), U(0, -1, 0), U(0, 1, 0), U(-1, 0, 0), U(1, 0, 0), U(0, 1, -1),
+             U(1, 0, -1), U(0, -1, 1), U(1, -1, 0), U(-1, 0, 1), U(-1, 1, 0));
   else {  // P3D27
     float sum = 0.0f;
     bool first = true;
@@ -202,6 +205,7 @@ extern "C" ot_status ot_naive_run(const ot_params* params, float* const* rw_a, f
       NAIVE(OT_G2D)
       NAIVE(OT_G3D)
       NAIVE(OT_TV2D)
+      NAIVE(OT_CS3D)
 #undef NAIVE
       case OT_RICIAN2D:
         hipLaunchKernelGGL(naive_rician_g, grid, block, 0, s, g, src[0], gbuf);

@@ -92,6 +92,7 @@ ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const 
       case OT_G3D: return launch_3dw_g3d(t_eff, g.wave_rows, g, f, stream, prepare);
       case OT_P3D7: return launch_3dw_p3d7(t_eff, g.wave_rows, g, f, stream, prepare);
       case OT_P3D27: return launch_3dw_p3d27(t_eff, g.wave_rows, g, f, stream, prepare);
+      case OT_CS3D: return launch_3dw_cs3d(t_eff, g.wave_rows, g, f, stream, prepare);
       default: return set_error(OT_EINVAL, "stencil %d is not a 3D stencil", stencil);
     }
   }
@@ -100,6 +101,7 @@ ot_status launch_3d(int stencil, int t_eff, int ept, const LaunchGeom& g, const 
     case OT_G3D: return launch_3d_g3d(t_eff, ept, g, f, stream, prepare);
     case OT_P3D7: return launch_3d_p3d7(t_eff, ept, g, f, stream, prepare);
     case OT_P3D27: return launch_3d_p3d27(t_eff, ept, g, f, stream, prepare);
+    case OT_CS3D: return set_error(OT_EUNSUPPORTED, "cs-3d has no LDS kernel; use a wave-pipelined tile");
     default: return set_error(OT_EINVAL, "stencil %d is not a 3D stencil", stencil);
   }
 }
@@ -121,6 +123,11 @@ ot_status kernel_shape(const ot_params& p, ot_derived& d) {
       bx = 256;  // four independent wavefronts per workgroup, no LDS
       by = 1;
       lds = 0;
+    } else if (p.stencil == OT_CS3D) {
+      return set_error(OT_EUNSUPPORTED,
+                       "cs-3d runs on the wave-pipelined 3D kernel only: tile x must be 64 - 2T (got %d at T=%d) "
+                       "and tile y + 2T one of 8, 12, 16, 20, 24, 32 rows within the register budget (got %d)",
+                       d.tile[0], p.time_tile, d.tile[1] + 2 * p.time_tile);
     } else {
       shape_3d(p.stencil, p.time_tile, p.elems_per_thread, d.tile, &bx, &by, &lds);
     }

@@ -5,15 +5,18 @@
 // the reference's stdout wire format:
 //   * flops: the constant in "Actual GFlop/s" (e.g. jacobi-2d.cpp:1001-1002,
 //     jacobi-3d.cpp:915-917, poisson-3d.cpp:883, gradient-3d.cpp:919,
-//     fdtd-2d.cpp:885-886, rician-denoise-2d.cpp:844-847, cdsc-tv-update-2d.cpp:569)
+//     fdtd-2d.cpp:885-886, rician-denoise-2d.cpp:844-847, cdsc-tv-update-2d.cpp:569,
+//     cs-3d.cpp:795-798)
 //   * pp_*: the (numArrays, loads, stores, flops) arguments of
 //     ProgramGenerator::printProgramParameters (jacobi-2d.cpp:737,
 //     poisson-2d.cpp:564, gradient-2d.cpp:606, fdtd-2d.cpp:643,
 //     rician-denoise-2d.cpp:647, cdsc-tv-update-2d.cpp:615, jacobi-1d.cpp:554);
 //     3D drivers print their own per-block keys with the constant 7
-//     (jacobi-3d.cpp:599-622, poisson-3d.cpp:565-603, gradient-3d.cpp:601-639)
+//     (jacobi-3d.cpp:599-622, poisson-3d.cpp:565-603, gradient-3d.cpp:601-639,
+//     cs-3d.cpp:595-630)
 //   * model features printed at the end of each driver (phase2_global_loads,
-//     compute_per_point, ... e.g. jacobi-2d.cpp:1029-1037).
+//     compute_per_point, ... e.g. jacobi-2d.cpp:1029-1037); cs-3d prints none,
+//     so its row counts its 13 taps and 65 flops.
 #pragma once
 #include <stdint.h>
 
@@ -35,7 +38,7 @@ struct StencilTraits {
 };
 
 // Indexed by ot_stencil.
-static const StencilTraits kTraits[13] = {
+static const StencilTraits kTraits[14] = {
   // name               dims r rw ro pad rule xpad fl  arrays/ld/st/fl   p2gl p2sh comp p3sh p4gs shst fld   n    bx  by bz
   {"jacobi-1d",          1, 1, 1, 0, 1, 0, 109,  3,  1, 3, 1, 3,          3, 0, 3, 3, 1, 1, 1,       1024, 16, 1, 1},
   {"jacobi-1d-5pt",      1, 2, 1, 0, 2, 0, 0,    5,  0, 5, 1, 5,          5, 0, 5, 5, 1, 1, 1,       1024, 16, 1, 1},
@@ -50,6 +53,7 @@ static const StencilTraits kTraits[13] = {
   {"gradient-3d",        3, 1, 1, 0, 1, 2, 0,    20, 0, 7, 1, 7,          7, 0, 20, 7, 1, 1, 1,      128, 8, 8, 8},
   {"rician-denoise-2d",  2, 2, 1, 1, 1, 1, 0,    42, 3, 6, 3, 44,         13, 0, 42, 13, 1, 1, 1,    1024, 16, 16, 1},
   {"cdsc-tv-update-2d",  2, 1, 1, 2, 1, 1, 0,    59, 1, 7, 1, 59,         7, 0, 59, 7, 1, 1, 1,      1024, 16, 16, 1},
+  {"cs-3d",              3, 1, 1, 0, 1, 2, 0,    65, 0, 7, 1, 7,          13, 0, 65, 13, 1, 1, 1,    128, 8, 8, 8},
 };
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

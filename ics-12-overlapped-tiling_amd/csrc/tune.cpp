@@ -152,7 +152,9 @@ ot_status ot_features(const ot_params* p, float* f) {
   if (st != OT_OK) return st;
   const ot::StencilTraits& tr = kTraits[p->stencil];
   for (int i = 0; i < OT_FEATURES; ++i) f[i] = 0.0f;
-  f[p->stencil] = 1.0f;
+  // one-hot over the 13 stencils the model was trained on; cs-3d has no slot
+  // and is placed by its descriptors (f[16], f[17], f[24..27]) alone
+  if (p->stencil < 13) f[p->stencil] = 1.0f;
   f[13] = (float)log2d(d.real_size[0]);
   f[14] = (float)log2d(d.real_size[1]);
   f[15] = (float)log2d(d.real_size[2]);

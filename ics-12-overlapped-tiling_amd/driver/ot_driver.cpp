@@ -354,11 +354,11 @@ int main(int argc, char** argv) {
 
   // model feature keys as each reference driver prints them (jacobi-2d.cpp:1029-1037,
   // jacobi-3d.cpp:919-926, fdtd-2d.cpp, rician-denoise-2d.cpp, cdsc-tv-update-2d.cpp:847-853)
-  static const double table[13][7] = {
+  static const double table[14][7] = {
       {3, 0, 3, 3, 1, 1, 1},   {5, 0, 5, 5, 1, 1, 1},    {7, 0, 7, 7, 1, 1, 1},   {5, 0, 5, 5, 1, 1, 1},
       {7, 0, 7, 5, 1, 1, 1},   {9, 0, 9, 9, 1, 1, 1},    {27, 0, 27, 27, 1, 1, 1}, {8, 0, 8, 6, 1, 1, 2},
       {7, 4, 11, 5, 3, 3, 3},  {5, 0, 12, 5, 1, 1, 1},   {7, 0, 20, 7, 1, 1, 1},  {13, 0, 42, 13, 1, 1, 1},
-      {7, 0, 59, 7, 1, 1, 1}};
+      {7, 0, 59, 7, 1, 1, 1},  {13, 0, 65, 13, 1, 1, 1}};
   const char* names[7] = {"phase2_global_loads", "phase2_shared_loads", "compute_per_point", "phase3_shared_loads",
                           "phase4_global_stores", "shared_stores", "num_fields"};
   for (int i = 0; i < 7; ++i) printValue(names[i], table[sid][i]);

@@ -19,9 +19,9 @@ LIB_PATH = os.path.join(HERE, "libot.so")
 STENCILS = [
     "jacobi-1d", "jacobi-1d-5pt", "jacobi-1d-7pt", "jacobi-2d", "jacobi-3d", "poisson-2d",
     "poisson-3d", "poisson-3d-rhs", "fdtd-2d", "gradient-2d", "gradient-3d",
-    "rician-denoise-2d", "cdsc-tv-update-2d",
+    "rician-denoise-2d", "cdsc-tv-update-2d", "cs-3d",
 ]
-(J1D3, J1D5, J1D7, J2D, J3D, P2D, P3D27, P3D7, FDTD2D, G2D, G3D, RICIAN2D, TV2D) = range(13)
+(J1D3, J1D5, J1D7, J2D, J3D, P2D, P3D27, P3D7, FDTD2D, G2D, G3D, RICIAN2D, TV2D, CS3D) = range(14)
 GEOM_TILE, GEOM_REFERENCE = 0, 1
 VARIANT_FDTD_PHYSICAL = 1
 

@@ -45,7 +45,8 @@ typedef enum ot_status {
 
 /* Stencils. Each one is the overlapped-tiled kernel of one reference driver
  * (opencl/<name>/<name>.cpp); P3D7 is the 7-point Poisson-Jacobi relaxation with
- * a read-only right-hand side that has no reference driver. */
+ * a read-only right-hand side that has no reference driver. CS3D runs on the
+ * wave-pipelined 3D kernel only (tile x = 64 - 2T, tile geometry). */
 typedef enum ot_stencil {
   OT_J1D3 = 0,    /* jacobi-1d        0.333f*(l+c+r)                       */
   OT_J1D5 = 1,    /* jacobi-1d-5pt    0.2f*sum5                            */
@@ -60,7 +61,8 @@ typedef enum ot_stencil {
   OT_G3D = 10,    /* gradient-3d      c + 1/sqrt(1e-4 + sum6 (c-n)^2)      */
   OT_RICIAN2D = 11, /* rician-denoise-2d, f read-only                      */
   OT_TV2D = 12,   /* cdsc-tv-update-2d, V and F read-only                  */
-  OT_NUM_STENCILS = 13
+  OT_CS3D = 13,   /* cs-3d            13-point gradient-weighted average   */
+  OT_NUM_STENCILS = 14
 } ot_stencil;
 
 enum {

@@ -93,7 +93,9 @@ ot_status ot_model_analytic(const ot_params* p, const ot_arch* arch, int32_t reg
 
 /* ---- learned cost model ------------------------------------------------------- */
 
-#define OT_FEATURES 28   /* one-hot stencil (13), log2 nx/ny/nz, radius, fields,
+#define OT_FEATURES 28   /* one-hot stencil (13: stencils 0..12; cs-3d is all
+                            zeros there and placed by its descriptors), log2
+                            nx/ny/nz, radius, fields,
                             T, log2 block x / rows / z, log2 ept, wave kernel,
                             dims, flops per point, loads per point, and the
                             natural log of ot_model_analytic's seconds per

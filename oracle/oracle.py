@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libot_oracle.so")
 
-DIMS = {0: 1, 1: 1, 2: 1, 3: 2, 4: 3, 5: 2, 6: 3, 7: 3, 8: 2, 9: 2, 10: 3, 11: 2, 12: 2}
+DIMS = {0: 1, 1: 1, 2: 1, 3: 2, 4: 3, 5: 2, 6: 3, 7: 3, 8: 2, 9: 2, 10: 3, 11: 2, 12: 2, 13: 3}
 RADIUS = {0: 1, 1: 2, 2: 3}
 NUM_RW = {8: 3}
 NUM_RO = {7: 1, 11: 1, 12: 2}

@@ -47,6 +47,7 @@
 #define OR_G3D 10
 #define OR_RICIAN2D 11
 #define OR_TV2D 12
+#define OR_CS3D 13
 
 typedef struct {
   int nx, ny, nz; /* interior extents (1 for unused axes) */

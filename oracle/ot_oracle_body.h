@@ -55,7 +55,11 @@ static void SFX(sweep_2d)(int which, const box_t* b, const REAL* a, REAL* o) {
  * poisson-3d.cpp:282-293 (GPU text; its CPU loop is a copy of Jacobi-3D):
  *   26c - sum of 26 neighbours in (dx, dy, dz) lexicographic order;
  * P3D7: the 7-point Poisson-Jacobi relaxation with right-hand side f,
- *   u' = (x- + x+ + y- + y+ + z- + z+ - f) / 6 (no reference counterpart). */
+ *   u' = (x- + x+ + y- + y+ + z- + z+ - f) / 6 (no reference counterpart);
+ * cs-3d.cpp:286-308 with the constants of :161-163 (GPU text; its CPU loop
+ *   :700-716 is a copy of Jacobi-3D). Taps as the on-chip steps address them
+ *   (:325-379): val_0_m1_0 = y-1, val_0_p1_0 = y+1 (the reference's first step,
+ *   :258-263, swaps those two). eps is the float 1e-9f, pow(d, 2) is d*d. */
 static void SFX(sweep_3d)(int which, const box_t* b, const REAL* a, const REAL* f, REAL* o) {
   const long sx = b->sx, sxy = b->sx * b->sy;
   for (int z = 0; z < b->nz; ++z)
@@ -69,6 +73,25 @@ static void SFX(sweep_3d)(int which, const box_t* b, const REAL* a, const REAL* 
           const REAL c = a[i];
           v = c + (REAL)1.0f / RS_ARG(((REAL)0.0001f + SQ(c - a[i - 1]) + SQ(c - a[i + 1]) + SQ(c - a[i - sx]) +
                                        SQ(c - a[i + sx]) + SQ(c - a[i - sxy]) + SQ(c - a[i + sxy])));
+        } else if (which == OR_CS3D) {
+          const REAL u0 = a[i], zm = a[i - sxy], zp = a[i + sxy], ym = a[i - sx], yp = a[i + sx];
+          const REAL xm = a[i - 1], xp = a[i + 1];
+          const REAL c1 = u0 * ((REAL)1.0f / RS_ARG((REAL)1e-9f + SQ(zp - u0) + SQ(yp - u0) + SQ(xp - u0))) /
+                          (REAL)1.1f;
+          const REAL c2 = u0 *
+                          ((REAL)1.0f / RS_ARG((REAL)1e-9f + SQ(u0 - zm) + SQ(a[i - sxy + sx] - zm) +
+                                               SQ(a[i - sxy + 1] - zm))) /
+                          (REAL)1.1f;
+          const REAL c3 = u0 *
+                          ((REAL)1.0f / RS_ARG((REAL)1e-9f + SQ(a[i + sxy - sx] - ym) + SQ(u0 - ym) +
+                                               SQ(a[i - sx + 1] - ym))) /
+                          (REAL)1.1f;
+          const REAL c4 = u0 *
+                          ((REAL)1.0f / RS_ARG((REAL)1e-9f + SQ(a[i + sxy - 1] - xm) + SQ(a[i + sx - 1] - xm) +
+                                               SQ(u0 - xm))) /
+                          (REAL)1.1f;
+          v = ((REAL)1.1f * u0 + c1 * (zp + yp + xp) + c2 * zm + c3 * ym + c4 * xm) /
+              ((REAL)1.1f + (REAL)3.0f * c1 + c2 + c3 + c4);
         } else if (which == OR_P3D7) {
           v = (a[i - 1] + a[i + 1] + a[i - sx] + a[i + sx] + a[i - sxy] + a[i + sxy] - f[i]) / (REAL)6.0f;
         } else { /* P3D27 */
@@ -160,7 +183,7 @@ static void SFX(step_fdtd)(const box_t* b, REAL* ex, REAL* ey, REAL* hz, int phy
 int SFX(oracle_run)(int stencil, int nx, int ny, int nz, int P, int steps, REAL** rw, const REAL** ro,
                     const float* coef, int variant) {
   int dims = (stencil <= OR_J1D7) ? 1 : ((stencil == OR_J3D || stencil == OR_P3D27 || stencil == OR_P3D7 ||
-                                          stencil == OR_G3D) ? 3 : 2);
+                                          stencil == OR_G3D || stencil == OR_CS3D) ? 3 : 2);
   box_t b = mkbox(dims, nx, ny, nz, P);
   const long n = nel(&b);
   if (stencil == OR_FDTD2D) {

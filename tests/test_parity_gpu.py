@@ -107,6 +107,13 @@ CASES_3D = [
     (otlib.J3D, (190, 44, 40), 5, 1, (62, 6, 8), 1),
     (otlib.G3D, (200, 40, 50), 5, 2, (60, 8, 12), 1),
     (otlib.P3D7, (200, 40, 60), 7, 2, (60, 4, 16), 1),
+    # cs-3d (wave-pipelined kernel only): boundary tiles, a shortened last
+    # launch, tiles inside the domain in x and y with steady-state blocks
+    (otlib.CS3D, (70, 30, 29), 5, 2, (60, 8, 8), 1),
+    (otlib.CS3D, (64, 30, 31), 7, 1, (62, 10, 8), 1),
+    (otlib.CS3D, (130, 40, 33), 8, 3, (58, 10, 16), 1),
+    (otlib.CS3D, (200, 40, 60), 7, 3, (58, 6, 16), 1),
+    (otlib.CS3D, (61, 29, 40), 9, 4, (56, 4, 16), 1),
 ]
 
 
