@@ -70,8 +70,11 @@ def shape_key(name, n3):
 
 
 def base_params(name, n3):
-    return otlib.make_params(name, nx=n3[0], ny=n3[1], nz=n3[2], problem_size=n3[0], steps=STEPS, T=1, x=16,
-                             y=4, z=4, e=1)
+    # the sweep's layout plan: a T=1 tile every stencil accepts (cs-3d plans
+    # only wave-pipelined tiles)
+    x, y = (62, 6) if name == "cs-3d" else (16, 4)
+    return otlib.make_params(name, nx=n3[0], ny=n3[1], nz=n3[2], problem_size=n3[0], steps=STEPS, T=1, x=x,
+                             y=y, z=4, e=1)
 
 
 def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float, stencils=None):
