@@ -54,6 +54,7 @@ def test_every_stencil_has_a_driver(otlib):
     ("fdtd-2d", ["-n", 300, "-t", 20, "-s", 5, "-x", 32, "-y", 16]),
     ("rician-denoise-2d", ["-n", 300, "-t", 20, "-s", 2, "-x", 32, "-y", 8]),
     ("cdsc-tv-update-2d", ["-n", 300, "-t", 20, "-s", 4, "-x", 32, "-y", 16, "--graph"]),
+    ("cs-3d", ["-n", 100, "-t", 9, "-s", 2, "-x", 60, "-y", 8, "-z", 8, "--geometry", "tile"]),
 ])
 def test_driver_verify_passes(name, args):
     rc, keys, r = run_driver(name, *args, "-v")

@@ -256,6 +256,8 @@ FULL_SIZE = [
     # the 2D wave kernel at the bench's tiles (mask-free path on interior tiles)
     (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 18, 1), 1), ((64, 4, 1), 4)),
     (otlib.J2D, (1024, 1024, 1), 64, (8, (48, 16, 1), 1), ((64, 4, 1), 4)),
+    # cs-3d plans only wave tiles, so its T=1 comparison is the bench's T=1 tile
+    (otlib.CS3D, (256, 256, 256), 8, (2, (60, 8, 16), 1), ((62, 6, 10), 1)),
 ]
 
 
