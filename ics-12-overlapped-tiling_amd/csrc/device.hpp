@@ -125,6 +125,20 @@ __device__ __forceinline__ float cs3d(float u0, float zm, float zp, float ym, fl
          ((((alpha + 3.0f * c1) + c2) + c3) + c4);
 }
 
+// The same update from the rsqrt of the four gradient sums, computed once per
+// point by the caller: g0 at the centre, gz, gy, gx at its z-1, y-1, x-1
+// neighbours (each the same expression and order as in cs3d above).
+__device__ __forceinline__ float cs3d_g(float u0, float zm, float zp, float ym, float yp, float xm, float xp,
+                                        float g0, float gz, float gy, float gx) {
+  const float sum_aref = 1.1f, alpha = 1.1f;
+  const float c1 = (u0 * g0) / sum_aref;
+  const float c2 = (u0 * gz) / sum_aref;
+  const float c3 = (u0 * gy) / sum_aref;
+  const float c4 = (u0 * gx) / sum_aref;
+  return ((((alpha * u0 + c1 * ((zp + yp) + xp)) + c2 * zm) + c3 * ym) + c4 * xm) /
+         ((((alpha + 3.0f * c1) + c2) + c3) + c4);
+}
+
 // cdsc-tv-update-2d.cpp:290-309. c3 duplicates c1 as in the reference (its
 // MATLAB comment repeats c1's formula); V and F read-only, alpha a parameter.
 __device__ __forceinline__ float tv2d(float tc, float tr, float ml, float mc, float mr, float bl,

@@ -118,6 +118,17 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
 
   float q[T][NQ][FR];
   float fq[T + 1][FR];
+  // cs-3d: rsqrt of the forward-difference gradient sum G of level s-1 at the
+  // plane stage s evaluates (slot PH % 2) and at the plane before it (the other
+  // slot). The four rsqrt arguments of the update (cs-3d.cpp:286-300) are G at
+  // the centre and at its z-1, y-1 and x-1 neighbours, each summed in the same
+  // order, so one rsqrt per point and level serves all four.
+  constexpr bool kShareG = ST == OT_CS3D;
+  float rg[kShareG ? T : 1][2][kShareG ? FR : 1];
+#pragma unroll
+  for (int l = 0; l < (kShareG ? T : 1); ++l)
+#pragma unroll
+    for (int i = 0; i < (kShareG ? FR : 1); ++i) rg[l][0][i] = rg[l][1][i] = 0.0f;
 #pragma unroll
   for (int l = 0; l < T; ++l)
 #pragma unroll
@@ -220,6 +231,19 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
       const int p = k - L * s;
       if constexpr (FULL) {
         if (s > s_end) break;
+      }
+      if constexpr (kShareG) {
+        // G of level s-1 at plane p, rows s-1 .. FR-1-s, before the plane
+        // checks: plane p+1's stage needs it as its z-1 term even when p
+        // itself is not evaluated (p = lo - 1 reads valid level s-1 planes)
+#pragma unroll
+        for (int i = s - 1; i <= FR - 1 - s; ++i) {
+          const float c = q[s - 1][Q1][i];
+          rg[s - 1][PH % 2][i] =
+              rsq(((1e-9f + sq(q[s - 1][QN][i] - c)) + sq(q[s - 1][Q1][i + 1] - c)) + sq(from_right(c) - c));
+        }
+      }
+      if constexpr (FULL) {
         const int lo = z0 - (T - s), hi = z0 + TZ - 1 + (T - s);
         if (p < lo || p > hi) continue;
         if (p < 0 || p >= g.n[2]) {
@@ -245,10 +269,10 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
           if constexpr (ST == OT_J3D) v = j3d(xm, xp, ym, yp, zm, zp, c);
           else if constexpr (ST == OT_G3D) v = g3d(xm, xp, ym, yp, zm, zp, c);
           else if constexpr (ST == OT_CS3D)
-            // diagonal taps: (z-1, y+1), (z-1, x+1), (z+1, y-1), (y-1, x+1),
-            // (z+1, x-1), (y+1, x-1)
-            v = cs3d(c, zm, zp, ym, yp, xm, xp, q[s - 1][Q2][i + 1], from_right(zm), q[s - 1][QN][i - 1],
-                     from_right(ym), from_left(zp), from_left(yp));
+            // the rsqrt of G at the centre, at z-1 (the previous plane's
+            // slot), at y-1 (row i-1) and at x-1 (the lane to the left)
+            v = cs3d_g(c, zm, zp, ym, yp, xm, xp, rg[s - 1][PH % 2][i], rg[s - 1][(PH + 1) % 2][i],
+                       rg[s - 1][PH % 2][i - 1], from_left(rg[s - 1][PH % 2][i]));
           else v = p3d7(xm, xp, ym, yp, zm, zp, fq[s][i]);
         } else {
           // poisson-3d.cpp:282-293: planes p-1, p, p+1 of level s-1 were
