@@ -151,6 +151,19 @@ __device__ __forceinline__ float tv2d(float tc, float tr, float ml, float mc, fl
          ((((alpha * f + c1 * bc) + c2 * tc) + c3 * mr) + c4 * ml);
 }
 
+// The same update from the rsqrt of the gradient sums, computed once per point
+// by the caller: g0 at the centre (c1 and its duplicate c3), gt at the top
+// neighbour (c2), gl at the left neighbour (c4), each the sum tv2d forms.
+__device__ __forceinline__ float tv2d_g(float tc, float ml, float mc, float mr, float bc, float v, float f,
+                                        float alpha, float g0, float gt, float gl) {
+  const float c1 = (mc * g0) / v;
+  const float c2 = (mc * gt) / v;
+  const float c3 = c1;
+  const float c4 = (mc * gl) / v;
+  return (1.0f / ((((alpha + c1) + c2) + c3) + c4)) *
+         ((((alpha * f + c1 * bc) + c2 * tc) + c3 * mr) + c4 * ml);
+}
+
 // rician-denoise-2d.cpp:154-162, :271-281, :322-327.
 struct RicianK {
   float dt, sigma2, gamma;

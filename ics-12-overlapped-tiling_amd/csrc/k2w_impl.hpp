@@ -129,6 +129,17 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
       }
       if constexpr (ST == OT_J2D || ST == OT_P2D || ST == OT_G2D || ST == OT_TV2D) {
         float nu[FR];
+        // TV: rsqrt of the gradient sum (1e-8 + (y+1 diff)^2) + (x+1 diff)^2 once
+        // per row; c1 (= c3), c2 and c4 are it at the row, the row above and the
+        // lane to the left (cdsc-tv-update-2d.cpp:294-300, same order of terms)
+        float gr[ST == OT_TV2D ? FR : 1];
+        if constexpr (ST == OT_TV2D) {
+#pragma unroll
+          for (int i = s - 1; i < FR - s; ++i) {
+            const float c = u[0][i];
+            gr[i] = rsq((1e-8f + sq(u[0][i + 1] - c)) + sq(from_right(c) - c));
+          }
+        }
 #pragma unroll
         for (int i = s; i < FR - s; ++i) {
           const float c = u[0][i], tc = u[0][i - 1], bc = u[0][i + 1];
@@ -142,8 +153,8 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
                     from_right(bc));
           } else {
             const float alpha = g.coef[0] != 0.0f ? g.coef[0] : 1.1f;
-            v = tv2d(tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc, ro[0][i], ro[1][i],
-                     alpha);
+            v = tv2d_g(tc, from_left(c), c, from_right(c), bc, ro[0][i], ro[1][i], alpha, gr[i], gr[i - 1],
+                       from_left(gr[i]));
           }
           nu[i] = mask(v, i);
         }
