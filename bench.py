@@ -59,7 +59,7 @@ WORKLOADS = {
     # compute-bound (4 rsqrt and 5 divisions per point): the sweep's best tile is
     # a T=1 tile, so the time-tiled and T=1 lines coincide
     "cs3d": dict(stencil="cs-3d", n=(256, 256, 256), steps=64,
-                 tiled=dict(T=1, x=62, y=6, z=10, e=1), t1=dict(T=1, x=62, y=6, z=10, e=1),
+                 tiled=dict(T=1, x=62, y=6, z=14, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),
                  bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
 }
 # batch workloads: (T, block x, block y rows, elements per thread) of the
@@ -67,7 +67,7 @@ WORKLOADS = {
 # per time tile) at the configuration that is fastest for per-image launches
 # (both from sweeps of the batch on an MI355X)
 DEFAULT_BATCH = {"batch-rician": (2, 64, 8, 4), "batch-tv": (3, 58, 26, 1)}
-PER_IMAGE_BEST = {"batch-rician": (4, 32, 8, 2), "batch-tv": (4, 32, 16, 1)}
+PER_IMAGE_BEST = {"batch-rician": (4, 32, 8, 2), "batch-tv": (5, 54, 6, 1)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jacobi3d_256_traffic.json")
 MODEL_FILE = os.path.join(ROOT, "ics-12-overlapped-tiling_amd", "tuning", "mi355x", "mlp.bin")
 
