@@ -352,8 +352,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this engine's path is single-GPU; "
-              f"see DESIGN.md for how the stencil would shard over several GPUs.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this engine's path is single-GPU "
+              f"(over several GPUs a grid would split into z-slabs with T-deep ghost planes).", file=sys.stderr)
         sys.exit(2)
     if args.workload.startswith("batch-"):
         T, bx, by, e = DEFAULT_BATCH[args.workload]

@@ -11,7 +11,7 @@ for p in (ROOT, PKG, os.path.join(ROOT, "oracle")):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on a machine with one)")
 
 
 @pytest.fixture(scope="session")

@@ -59,7 +59,7 @@ def test_reference_geometry_examples(otlib):
     d = otlib.derive(otlib.make_params("jacobi-2d", n=6000, x=32, y=4, e=1, T=1, geometry="reference"))
     assert (d.real_size[0], d.num_blocks[0], d.num_blocks[1], d.padded_size[0]) == (6000, 187, 1500, 6002)
     assert d.grid_blocks[0] == 188  # this engine also computes the columns the reference leaves out
-    # BASELINE config 1 in reference geometry: 1020^2, 16 launches
+    # Jacobi-2D 1024^2, 64 steps, T=4, 16x16 in reference geometry: 1020^2, 16 launches
     d = otlib.derive(otlib.make_params("jacobi-2d", n=1024, x=16, y=16, e=1, T=4, t=64, geometry="reference"))
     assert (d.real_size[0], d.num_blocks[0], d.launches, d.padding) == (1020, 102, 16, 4)
     # jacobi-1d adds 109 to the padded size
