@@ -40,7 +40,7 @@ typedef enum ot_status {
                            (reference: exit 1, opencl/jacobi-2d/jacobi-2d.cpp:682-690) */
   OT_EHIP = 3,          /* HIP runtime error (reference: CLContext::throwOnError,
                            opencl/common/src/CLContext.cpp:50-56) */
-  OT_EUNSUPPORTED = 4   /* configuration this build does not implement */
+  OT_EUNSUPPORTED = 4   /* configuration the engine does not implement */
 } ot_status;
 
 /* Stencils. Each one is the overlapped-tiled kernel of one reference driver
