@@ -150,8 +150,7 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
     } else if constexpr (ST == OT_RICIAN2D) {
       // plane 2 holds g; planes 0/1 ping-pong u
       RicianK rk;
-      const float sigma = g.coef[0];
-      rk.sigma2 = sigma != 0.0f ? sigma * sigma : 0.0025f;
+      rk.sigma2 = g.coef[0] != 0.0f ? g.coef[0] : 0.0025f;  // SIGMA2 (rician-denoise-2d.cpp:159)
       rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
       rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
       float gg[E];

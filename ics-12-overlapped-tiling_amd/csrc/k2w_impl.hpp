@@ -162,8 +162,7 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
         for (int i = s; i < FR - s; ++i) u[0][i] = nu[i];
       } else if constexpr (ST == OT_RICIAN2D) {
         RicianK rk;
-        const float sigma = g.coef[0];
-        rk.sigma2 = sigma != 0.0f ? sigma * sigma : 0.0025f;
+        rk.sigma2 = g.coef[0] != 0.0f ? g.coef[0] : 0.0025f;  // SIGMA2 (rician-denoise-2d.cpp:159)
         rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
         rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
         // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]

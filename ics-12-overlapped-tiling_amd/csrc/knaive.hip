@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) naive_rician_u(NaiveGeom g, const float* 
   const int x = (int)(id % g.n[0]), y = (int)(id / g.n[0]);
   const int64_t i = g.origin + (int64_t)y * g.sy + x;
   RicianK rk;
-  rk.sigma2 = g.coef[0] != 0.0f ? g.coef[0] * g.coef[0] : 0.0025f;
+  rk.sigma2 = g.coef[0] != 0.0f ? g.coef[0] : 0.0025f;  // SIGMA2 (rician-denoise-2d.cpp:159)
   rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
   rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
   out[i] = rician_u(u[i], f[i], tap(u, g, x - 1, y, 0), tap(u, g, x + 1, y, 0), tap(u, g, x, y - 1, 0),

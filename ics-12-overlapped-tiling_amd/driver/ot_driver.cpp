@@ -70,7 +70,7 @@ int usage(const char* prog) {
                "  --geometry arg (=reference)        reference | tile\n"
                "  --nx / --ny / --nz arg             Per-axis extents (tile geometry)\n"
                "  --graph                            Replay the time loop from a hipGraph\n"
-               "  --alpha / --sigma / --lambda / --dt arg   Stencil scalars (TV alpha; Rician)\n"
+               "  --alpha / --sigma2 / --lambda / --dt arg  Stencil scalars (TV alpha; Rician SIGMA2, LAMBDA, DT)\n"
                "  --fdtd-physical                    Hz update with the curl sign\n"
                "  --seed arg (=123456)               srand() seed of the input\n";
   return 1;
@@ -128,7 +128,7 @@ int parse(int argc, char** argv, Options& o) {
     else if (key == "nz") p.nz = std::stoi(val);
     else if (key == "graph") o.graph = true;
     else if (key == "alpha") p.coef[0] = std::stof(val);
-    else if (key == "sigma") p.coef[0] = std::stof(val);
+    else if (key == "sigma2") p.coef[0] = std::stof(val);
     else if (key == "lambda") p.coef[1] = std::stof(val);
     else if (key == "dt") p.coef[2] = std::stof(val);
     else if (key == "fdtd-physical") p.variant |= OT_VARIANT_FDTD_PHYSICAL;

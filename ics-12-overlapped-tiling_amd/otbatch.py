@@ -107,7 +107,7 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
     w, h = draw_sizes(n_images, seed=seed)
     ims, total = layout(w, h)
     f = phantom_arena(ims, total, seed=seed, sigma=0.05)
-    coef = [0.05, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
+    coef = [0.0025, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
     ro = [f] if st == otlib.RICIAN2D else [np.ones(total, np.float32), f]
     A, B = DeviceBuffer.from_array(f), DeviceBuffer.from_array(f)
     RO = [DeviceBuffer.from_array(x) for x in ro]

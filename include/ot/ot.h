@@ -105,8 +105,10 @@ typedef struct ot_params {
   int32_t variant;          /* OT_VARIANT_* flags */
   float coef[4];            /* stencil scalars, 0 = reference default:
                                TV2D: coef[0] = alpha (1.1, cdsc-tv-update-2d.cpp:164)
-                               RICIAN2D: coef[0] = sigma (0.05), coef[1] = lambda
-                               (0.065), coef[2] = dt (5.0), rician-denoise-2d.cpp:154-161 */
+                               RICIAN2D: coef[0] = sigma^2 (SIGMA2, the literal
+                               0.0025), coef[1] = lambda (0.065), coef[2] = dt (5.0);
+                               gamma = lambda / sigma^2 in FP32,
+                               rician-denoise-2d.cpp:153-160 */
 } ot_params;
 
 typedef struct ot_derived {

@@ -27,22 +27,28 @@ NUM_RO = {7: 1, 11: 1, 12: 2}
 _lib = None
 
 
+def load(path: str):
+    """Load an oracle library build (the shipped one, or a copy built from
+    edited sources by the tests that check the pins catch planted mistakes)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built: make -C {HERE}")
+    L = C.CDLL(path)
+    for suffix, ct in (("f32", C.c_float), ("f64", C.c_double)):
+        fn = getattr(L, f"oracle_run_{suffix}")
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_int] * 6 + [C.POINTER(C.POINTER(ct)), C.POINTER(C.POINTER(ct)),
+                                      C.POINTER(C.c_float), C.c_int]
+    L.oracle_g2d_double_sqrt.restype = None
+    L.oracle_g2d_double_sqrt.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_float)]
+    L.oracle_reference_init.restype = None
+    L.oracle_reference_init.argtypes = [C.c_int] * 6 + [C.c_uint, C.POINTER(C.c_float)]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} not built: make -C {HERE}")
-        L = C.CDLL(LIB_PATH)
-        for suffix, ct in (("f32", C.c_float), ("f64", C.c_double)):
-            fn = getattr(L, f"oracle_run_{suffix}")
-            fn.restype = C.c_int
-            fn.argtypes = [C.c_int] * 6 + [C.POINTER(C.POINTER(ct)), C.POINTER(C.POINTER(ct)),
-                                          C.POINTER(C.c_float), C.c_int]
-        L.oracle_g2d_double_sqrt.restype = None
-        L.oracle_g2d_double_sqrt.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_float)]
-        L.oracle_reference_init.restype = None
-        L.oracle_reference_init.argtypes = [C.c_int] * 6 + [C.c_uint, C.POINTER(C.c_float)]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
 
 
@@ -62,17 +68,19 @@ def unpad(box: np.ndarray, dims: int, P: int) -> np.ndarray:
 
 
 def run(stencil: int, rw, ro=(), steps: int = 1, coef=(0, 0, 0, 0), variant: int = 0,
-        dtype=np.float32, timed: bool = False):
+        dtype=np.float32, timed: bool = False, library=None):
     """Advance `steps` time steps. rw/ro: lists of (nz, ny, nx) interior arrays.
     Returns the list of rw interiors after the steps (and the seconds spent in
-    the C loop when timed=True)."""
+    the C loop when timed=True). library: an oracle build from load() (default:
+    the shipped one)."""
     dims = DIMS[stencil]
     P = max(RADIUS.get(stencil, 1), 1)
     nz, ny, nx = rw[0].shape
     boxes = [pad(np.asarray(a, dtype=dtype), dims, P) for a in rw]
     roboxes = [pad(np.asarray(a, dtype=dtype), dims, P) for a in ro]
     ct = C.c_float if dtype == np.float32 else C.c_double
-    fn = lib().oracle_run_f32 if dtype == np.float32 else lib().oracle_run_f64
+    L = library if library is not None else lib()
+    fn = L.oracle_run_f32 if dtype == np.float32 else L.oracle_run_f64
     rwp = (C.POINTER(ct) * 3)(*[b.ctypes.data_as(C.POINTER(ct)) for b in boxes])
     rop = (C.POINTER(ct) * 2)(*[b.ctypes.data_as(C.POINTER(ct)) for b in roboxes])
     cf = (C.c_float * 4)(*[float(c) for c in (list(coef) + [0, 0, 0, 0])[:4]])

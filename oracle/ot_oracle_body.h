@@ -179,7 +179,7 @@ static void SFX(step_fdtd)(const box_t* b, REAL* ex, REAL* ey, REAL* hz, int phy
 
 /* Run `steps` time steps of `stencil` in place.
  *   rw: num_rw arrays (FDTD: ex, ey, hz), ro: read-only arrays (TV: V, F;
- *   Rician: f; P3D7: f), coef: TV alpha / Rician (sigma, lambda, dt), 0 = default. */
+ *   Rician: f; P3D7: f), coef: TV alpha / Rician (sigma^2, lambda, dt), 0 = default. */
 int SFX(oracle_run)(int stencil, int nx, int ny, int nz, int P, int steps, REAL** rw, const REAL** ro,
                     const float* coef, int variant) {
   int dims = (stencil <= OR_J1D7) ? 1 : ((stencil == OR_J3D || stencil == OR_P3D27 || stencil == OR_P3D7 ||
@@ -203,8 +203,8 @@ int SFX(oracle_run)(int stencil, int nx, int ny, int nz, int P, int steps, REAL*
       return -1;
     }
   }
-  const REAL sigma = coef ? (REAL)coef[0] : 0;
-  const REAL sigma2 = sigma != 0 ? (REAL)((float)coef[0] * (float)coef[0]) : (REAL)0.0025f;
+  /* Rician: coef[0] is SIGMA2 itself (the literal 0.0025f of :159 by default) */
+  const REAL sigma2 = (coef && coef[0] != 0.0f) ? (REAL)coef[0] : (REAL)0.0025f;
   const REAL lambda = (coef && coef[1] != 0.0f) ? (REAL)coef[1] : (REAL)0.065f;
   const REAL gamma = (REAL)((float)lambda / (float)sigma2);
   const REAL dt = (coef && coef[2] != 0.0f) ? (REAL)coef[2] : (REAL)5.0f;

@@ -196,7 +196,7 @@ def test_tv_and_rician_parameters(otlib):
     p = _params(otlib.TV2D, (50, 40, 1), 6, 3, (16, 16, 1), 1, coef=[0.1, 0, 0, 0])
     rw, ro = random_inputs(otlib.TV2D, (1, 40, 50), seed=5)
     check_against_oracle(p, rw, ro)
-    p = _params(otlib.RICIAN2D, (50, 40, 1), 6, 3, (16, 16, 1), 1, coef=[0.05, 0.1, 2.0, 0])
+    p = _params(otlib.RICIAN2D, (50, 40, 1), 6, 3, (16, 16, 1), 1, coef=[0.003, 0.1, 2.0, 0])
     rw, ro = random_inputs(otlib.RICIAN2D, (1, 40, 50), seed=6)
     check_against_oracle(p, rw, ro)
 
