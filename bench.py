@@ -3,7 +3,7 @@
 
 Workload (default): Jacobi-3D 7-point FP32 on a 256^3 grid, 32 time steps,
 interior values uniform [0,1) from a seed, Dirichlet-0 boundary, time-tiled with
-the tile configuration below. One bench "step" is one whole 32-time-step solve
+the autotuned tile configuration of workloads.py. One bench "step" is one whole 32-time-step solve
 of the grid (ceil(32/T) kernel launches), with the fields resident in HBM; the
 K timed solves are enqueued back to back between two device synchronisations,
 once as hipGraph replays and once as eager launches, and the faster of the two
@@ -12,11 +12,13 @@ is the headline (both are reported).
 Prints one JSON line with the metric (GStencil/s = grid points x time steps /
 second), the roofline of the dominant kernel (algorithmic HBM bytes per launch
 = 8 B x points: each launch reads and writes every point once, divided by the
-average launch duration measured with HIP events on the launch stream), the
-same workload at T=1 for the time-tiling gain, the autotuning leg (exhaustive
-sweep of the workload's configuration space against the learned cost model's
-pick and the analytic model's pick) and the CPU baseline (the oracle's
-restatement of the reference CPU loop, single-threaded, on the same workload).
+average launch duration: HIP events on the launch stream around the timed
+solves, over the launches), the same workload at T=1 for the time-tiling gain,
+the autotuning leg (exhaustive sweep of the workload's configuration space
+against the learned cost model's pick and the analytic model's pick), the CPU
+baseline (the oracle's restatement of the reference CPU loop, single-threaded,
+on the same workload) and, from that same oracle run, the parity of one GPU
+solve: bitwise against the FP32 oracle and its error against the FP64 oracle.
 """
 import argparse
 import json
@@ -31,7 +33,8 @@ sys.path.insert(0, os.path.join(ROOT, "ics-12-overlapped-tiling_amd"))
 import numpy as np  # noqa: E402
 
 import otlib  # noqa: E402
-from workloads import DEFAULT_BATCH, PER_IMAGE_BEST, WORKLOADS, make_inputs, plan_params  # noqa: E402
+from workloads import (DEFAULT_BATCH, PER_IMAGE_BEST, WORKLOADS, batch_inputs, make_inputs,  # noqa: E402
+                       plan_params)
 
 HBM_PEAK_GBS = 8000.0
 
@@ -74,12 +77,13 @@ def calibrate_copy(mib):
 
 
 def solve_rate(w, cfg, n3, steps, K, W):
-    """Time K whole solves after W warm-ups, each solve being the ceil(steps/T)
-    launches replayed from the plan's hipGraph, enqueued back to back (one
-    device synchronisation before and after the K solves). Returns (seconds
-    per solve, average launch seconds from HIP events on the launch stream,
-    launches per solve, eager per-launch average, derived geometry, seconds
-    per solve of the same loop with one synchronous solve per call)."""
+    """Time K whole solves after W warm-ups, enqueued back to back between two
+    device synchronisations, once as hipGraph replays (on the plan's stream)
+    and once as eager launches (on the null stream). Each timed region is also
+    bracketed by a HIP event pair on the stream the kernels run on. Returns a
+    dict: per mode the host seconds per solve and the event seconds per solve;
+    the eager kernel-only seconds per launch (one event pair per launch);
+    launches per solve and the derived geometry."""
     p = plan_params(w, cfg, n3, steps)
     plan = otlib.Plan(p)
     d = plan.derived
@@ -87,62 +91,62 @@ def solve_rate(w, cfg, n3, steps, K, W):
     A = [otlib.DeviceBuffer.from_array(x) for x in rw]
     B = [otlib.DeviceBuffer.from_array(x) for x in rw]
     R = [otlib.DeviceBuffer.from_array(x) for x in ro]
-    for _ in range(W):
-        plan.run(A, B, R, use_graph=True)
-    otlib.check(otlib.lib().ot_dev_sync())
-    t0 = time.perf_counter()
-    for _ in range(K):
-        plan.run(A, B, R, use_graph=True, enqueue_only=True)
-    otlib.check(otlib.lib().ot_dev_sync())
-    t1 = time.perf_counter()
-    # the same loop from eager launches, back to back
-    t4 = time.perf_counter()
-    for _ in range(K):
-        plan.run(A, B, R, enqueue_only=True)
-    otlib.check(otlib.lib().ot_dev_sync())
-    eager_back_to_back = (time.perf_counter() - t4) / K
-    # the graph replays one at a time, bracketed by HIP events on the graph's stream
-    ev_total = 0.0
-    t2 = time.perf_counter()
-    for _ in range(K):
-        tm = plan.run(A, B, R, use_graph=True)
-        ev_total += tm.event_s
-    t3 = time.perf_counter()
-    launches = d.launches
-    # eager pass with one event pair per launch (cross-check of the launch time)
+    out = {"launches": d.launches, "derived": d}
+    for mode, graph, stream in (("graph", True, plan.stream()), ("eager", False, None)):
+        for _ in range(W):
+            plan.run(A, B, R, use_graph=graph)
+        timer = otlib.StreamTimer(stream)
+        otlib.check(otlib.lib().ot_dev_sync())
+        t0 = time.perf_counter()
+        timer.start()
+        for _ in range(K):
+            plan.run(A, B, R, use_graph=graph, enqueue_only=True)
+        ev = timer.stop()
+        otlib.check(otlib.lib().ot_dev_sync())
+        out[mode] = {"wall_s": (time.perf_counter() - t0) / K, "event_s": ev / K}
+        timer.close()
+    # eager pass with one event pair per launch: kernel time without the gaps
     eager = []
     for _ in range(max(2, min(K, 5))):
         tm = plan.run(A, B, R, time_launches=True)
         eager.append(tm.kernel_s / tm.launches)
+    out["kernel_launch_s"] = float(np.median(eager))
     plan.close()
     for buf in A + B + R:
         buf.free()
-    return ((t1 - t0) / K, ev_total / (K * launches), launches, float(np.median(eager)), d, (t3 - t2) / K,
-            eager_back_to_back)
+    return out
 
 
-def cpu_baseline(w, n3, steps):
+def cpu_baseline(w, n3, steps, tiled):
     """The oracle (C restatement of the reference CPU loop) on the same
-    workload, single-threaded on the host."""
+    workload, single-threaded on the host, and the checker of the GPU result:
+    one GPU solve from the same inputs at the headline configuration against
+    the FP32 oracle (bitwise) and the FP64 oracle (relative error)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     st = otlib.stencil_id(w["stencil"])
-    # the layout of the workload's T=1 plan (a tile every stencil accepts)
-    d = otlib.derive(plan_params(w, w["t1"], n3, steps))
+    p = plan_params(w, tiled, n3, steps)
+    d = otlib.derive(p)
     rw, ro = make_inputs(w, d)
     sl = otlib.interior_slices(d)
+    rw_i, ro_i = [x[sl].copy() for x in rw], [x[sl].copy() for x in ro]
     # single-threaded, pinned to one core of the host for the duration of the run
     allowed = os.sched_getaffinity(0)
     core = min(allowed)
     os.sched_setaffinity(0, {core})
     try:
-        _, dt = oracle.run(st, [x[sl].copy() for x in rw], [x[sl].copy() for x in ro], steps=steps, timed=True)
+        ref, dt = oracle.run(st, [x.copy() for x in rw_i], ro_i, steps=steps, timed=True)
     finally:
         os.sched_setaffinity(0, allowed)
+    gpu = [x[sl] for x in otlib.run_host(p, rw, ro).fields]
+    m32 = [oracle.compare(r, g) for r, g in zip(ref, gpu)]
+    ref64 = oracle.run(st, [x.astype(np.float64) for x in rw_i], [x.astype(np.float64) for x in ro_i],
+                       steps=steps, dtype=np.float64)
+    m64 = [oracle.compare(r, g) for r, g in zip(ref64, gpu)]
     pts = float(n3[0]) * n3[1] * n3[2]
     shape = "x".join(str(v) for v in n3 if v > 1)
-    return {
+    base = {
         "value": round(pts * steps / dt / 1e9, 4),
         "unit": "GStencil/s",
         "cores": 1,
@@ -151,6 +155,12 @@ def cpu_baseline(w, n3, steps):
                   f"reference CPU loop, gcc -O2 -ffp-contract=off, 1 thread pinned to core {core} of "
                   f"{len(allowed)} available ({_cpu_model()}), {dt:.2f} s",
     }
+    parity = {"bitwise_vs_fp32_oracle": all(m["bitwise_equal"] for m in m32),
+              "max_rel_vs_fp32_oracle": max(m["max_rel"] for m in m32),
+              "fp64_max_rel": max(m["max_rel"] for m in m64),
+              "fp64_l2_rel": max(m["l2_rel"] for m in m64),
+              "checked": f"one {steps}-step GPU solve at the headline configuration, every field, all points"}
+    return base, parity
 
 
 def _cpu_model():
@@ -173,15 +183,12 @@ def bench_batch(args, stencil_name):
     import otbatch
 
     st = otlib.RICIAN2D if stencil_name == "rician" else otlib.TV2D
-    widths, heights = otbatch.draw_sizes(args.images, seed=77)
-    ims, total = otbatch.layout(widths, heights)
-    f = otbatch.phantom_arena(ims, total, seed=77, sigma=0.05)
-    coef = [0.0025, 0.065, 5.0, 0.0] if st == otlib.RICIAN2D else [0.1, 0.0, 0.0, 0.0]
+    ims, total, widths, heights, rw_host, ro_host, coef = batch_inputs("batch-" + stencil_name, args.images)
+    f = rw_host[0]
     T, bx, by, e = (args.T, args.bx, args.by, args.ept)
     p = otlib.make_params(st, steps=args.iterations, T=T, x=bx, y=by, e=e, coef=coef)
     Tq, bxq, byq, eq = PER_IMAGE_BEST["batch-" + stencil_name]
     q = otlib.make_params(st, steps=args.iterations, T=Tq, x=bxq, y=byq, e=eq, coef=coef)
-    ro_host = [f] if st == otlib.RICIAN2D else [np.ones(total, np.float32), f]
     A = otlib.DeviceBuffer.from_array(f)
     B = otlib.DeviceBuffer.from_array(f)
     RO = [otlib.DeviceBuffer.from_array(x) for x in ro_host]
@@ -317,15 +324,19 @@ def main():
     if args.T:
         tiled = dict(T=args.T, x=args.bx, y=args.by, z=args.bz or 1, e=args.ept or 1)
 
-    sec_graph, launch_s, launches, eager_launch_s, d, sec_sync, sec_eager = solve_rate(w, tiled, n3, steps_t,
-                                                                                       args.steps, args.warmup)
+    r = solve_rate(w, tiled, n3, steps_t, args.steps, args.warmup)
+    d, launches = r["derived"], r["launches"]
     # back-to-back solves from graph replays or from eager launches: the faster
-    # of the two is the headline, both are reported
-    mode = "graph" if sec_graph <= sec_eager else "eager"
-    sec = min(sec_graph, sec_eager)
+    # of the two (host clock around the K solves) is the headline, both are
+    # reported
+    mode = "graph" if r["graph"]["wall_s"] <= r["eager"]["wall_s"] else "eager"
+    sec = r[mode]["wall_s"]
     points = float(n3[0]) * n3[1] * n3[2]
     gst = points * steps_t / sec / 1e9
     bytes_per_launch = float(w["bytes_pp"]) * points
+    # the headline mode's HIP-event time over the K solves on the kernels'
+    # stream, per launch (gaps between launches included)
+    launch_s = r[mode]["event_s"] / launches
     achieved = bytes_per_launch / launch_s / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -349,12 +360,14 @@ def main():
         "config": {"workload": f"{w['stencil']} {shape}, {steps_t} time steps, time-tiled",
                    "time_tile": tiled["T"], "tile": [tiled["x"], tiled["y"] * tiled["e"], tiled["z"]],
                    "elements_per_thread": tiled["e"], "launches_per_step": launches, "launch_mode": mode,
-                   "ms_per_step_graph": round(sec_graph * 1e3, 4), "ms_per_step_eager": round(sec_eager * 1e3, 4),
-                   "ms_per_step_graph_synchronous": round(sec_sync * 1e3, 4)},
+                   "ms_per_step_graph": round(r["graph"]["wall_s"] * 1e3, 4),
+                   "ms_per_step_eager": round(r["eager"]["wall_s"] * 1e3, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel_name(w, d), "avg_launch_us": round(launch_s * 1e6, 2),
-                     "eager_avg_launch_us": round(eager_launch_s * 1e6, 2),
+                     "timing": f"HIP events on the {mode} stream around the {args.steps} timed solves, "
+                               f"/ ({args.steps} x {launches} launches)",
+                     "kernel_only_avg_launch_us": round(r["kernel_launch_s"] * 1e6, 2),
                      "bytes_per_launch": bytes_per_launch},
         # the reference drivers' "Actual GFlop/s": useful points x flops per
         # point x steps / time (jacobi-2d.cpp:903-1013)
@@ -362,17 +375,18 @@ def main():
     }
     if not args.no_t1:
         t1 = w["t1"]
-        sec1g, launch1, l1, _, _, _, sec1e = solve_rate(w, t1, n3, steps_t, max(3, args.steps // 2), args.warmup)
-        sec1 = min(sec1g, sec1e)
+        r1 = solve_rate(w, t1, n3, steps_t, args.steps, args.warmup)
+        m1 = "graph" if r1["graph"]["wall_s"] <= r1["eager"]["wall_s"] else "eager"
+        sec1 = r1[m1]["wall_s"]
         g1 = points * steps_t / sec1 / 1e9
-        out["t1"] = {"value": round(g1, 2), "ms_per_step": round(sec1 * 1e3, 4),
-                     "achieved_GBs": round(bytes_per_launch / launch1 / 1e9, 1),
+        out["t1"] = {"value": round(g1, 2), "ms_per_step": round(sec1 * 1e3, 4), "launch_mode": m1,
+                     "achieved_GBs": round(bytes_per_launch / (r1[m1]["event_s"] / r1["launches"]) / 1e9, 1),
                      "tile": [t1["x"], t1["y"] * t1["e"], t1["z"]]}
         out["speedup_vs_t1"] = round(gst / g1, 3)
     if not args.no_tune:
         out["tuning"] = tuning_leg(w, n3, steps_t, w["tiled"])
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, n3, steps_t)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(w, n3, steps_t, tiled)
     print(json.dumps(out), flush=True)
 
 

@@ -88,10 +88,15 @@ __device__ __forceinline__ float p2d(float tl, float tc, float tr, float ml, flo
 __device__ __forceinline__ float g2d(float l, float c, float r, float t, float b) {
   return c + rsq((((0.0001f + sq(c - l)) + sq(c - r)) + sq(c - t)) + sq(c - b));
 }
-// jacobi-3d.cpp:734-741: order (y-1, c, y+1, x-1, x+1, z+1, z-1)
+// jacobi-3d.cpp:734-741: order (y-1, c, y+1, x-1, x+1, z+1, z-1); j3d() is
+// 0.143f times this sum
+__device__ __forceinline__ float j3d_sum(float xm, float xp, float ym, float yp, float zm, float zp,
+                                         float c) {
+  return (((((ym + c) + yp) + xm) + xp) + zp) + zm;
+}
 __device__ __forceinline__ float j3d(float xm, float xp, float ym, float yp, float zm, float zp,
                                      float c) {
-  return 0.143f * ((((((ym + c) + yp) + xm) + xp) + zp) + zm);
+  return 0.143f * j3d_sum(xm, xp, ym, yp, zm, zp, c);
 }
 // gradient-3d.cpp:305-307: val0..val6 = x-1, c, x+1, y-1, y+1, z-1, z+1
 __device__ __forceinline__ float g3d(float xm, float xp, float ym, float yp, float zm, float zp,

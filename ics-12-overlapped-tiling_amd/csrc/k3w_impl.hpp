@@ -109,6 +109,17 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
     row_st[i] = (row_in && i >= T && i < T + TY) ? o : range;
     mxy[i] = (in_x && row_in) ? 0xFFFFFFFFu : 0u;
   }
+  // jacobi-3d: the Dirichlet mask rides on the update's multiply. A point in
+  // the domain takes sum * 0.143f, the very product of the update; a point
+  // outside it takes sum * 0, a zero for every finite sum, where the other
+  // stencils AND their result with the mask. This saves one VALU instruction
+  // of the eight per evaluation. (A zero's sign can differ from the +0 of the
+  // reference's ring only when the sum is negative; it then changes no
+  // neighbour's bits unless that neighbour's seven terms are all zeros.)
+  constexpr bool kMulMask = ST == OT_J3D;
+  float cmul[kMulMask ? FR : 1];
+#pragma unroll
+  for (int i = 0; i < (kMulMask ? FR : 1); ++i) cmul[i] = mxy[i] ? 0.143f : 0.0f;
   if (g.phase_limit == 1) return;
   // lane + row term of every footprint row, for the steady-state loads
   constexpr bool kSoff = kSteady && T >= 2;
@@ -266,7 +277,7 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
           const float ym = q[s - 1][Q1][i - 1];
           const float yp = q[s - 1][Q1][i + 1];
           const float xm = from_left(c), xp = from_right(c);
-          if constexpr (ST == OT_J3D) v = j3d(xm, xp, ym, yp, zm, zp, c);
+          if constexpr (kMulMask) v = j3d_sum(xm, xp, ym, yp, zm, zp, c) * cmul[i];
           else if constexpr (ST == OT_G3D) v = g3d(xm, xp, ym, yp, zm, zp, c);
           else if constexpr (ST == OT_CS3D)
             // the rsqrt of G at the centre, at z-1 (the previous plane's
@@ -294,7 +305,7 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
               }
           v = 26.0f * Pz[1][i] - sum;
         }
-        const float nv = __uint_as_float(__float_as_uint(v) & mxy[i]);
+        const float nv = kMulMask ? v : __uint_as_float(__float_as_uint(v) & mxy[i]);
         if (s < T) q[s][QN][i] = nv;
         else outv[i] = nv;
       }

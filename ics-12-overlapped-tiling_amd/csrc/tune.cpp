@@ -53,10 +53,13 @@ void candidates(const ot_params& base, int dims, std::vector<Cand>& out) {
     for (int T = 1; T <= (stencil == OT_RICIAN2D ? 5 : 8); ++T)
       for (int rows : {64, 128, 256, 512}) out.push_back({T, 64 - 2 * R * T, rows, 1, 1});
   } else {
-    for (int T = 1; T <= 6; ++T)
-      for (int bx : {16, 32, 64})
-        for (int by : {2, 4, 8})
-          for (int bz : {8, 16, 32})
+    // LDS kernel: T 1..8 and tiles from 8 x 8 x 4 to 64 x 16 x 32 (x, y rows
+    // of ept, z-chunk), which holds the reference's J3D sweep grid
+    // (scripts/c2050/j3d.yml:7-13: block 8..32 x 2..8 x 2..8, T 1..4)
+    for (int T = 1; T <= 8; ++T)
+      for (int bx : {8, 16, 32, 64})
+        for (int by : {2, 4, 8, 16})
+          for (int bz : {4, 8, 16, 32})
             for (int e : {1, 2, 4, 8})
               if (e < 8 || T <= 2) out.push_back({T, bx, by, bz, e});
     // wave-pipelined kernel: x tile + 2T fills the 64 lanes, footprint rows FR.

@@ -341,6 +341,12 @@ class Plan:
         check(lib().ot_run(self.handle, a, b, r, stream, mode, C.byref(tm)))
         return tm
 
+    def stream(self):
+        """The plan's own stream, on which graph replays with stream=None run."""
+        h = C.c_void_p()
+        check(lib().ot_plan_stream(self.handle, C.byref(h)))
+        return h.value
+
     def launch(self, src, dst, ro=(), base_time=0, stream=None) -> None:
         s = _ptr_array([x.ptr for x in src])
         d = _ptr_array([x.ptr for x in dst])
@@ -357,6 +363,50 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class StreamTimer:
+    """A HIP event pair on one stream (None = the null stream): the time the GPU
+    spends on the work enqueued on that stream between start() and stop().
+    Events come from the HIP runtime libot.so itself links."""
+
+    _hip = None
+
+    def __init__(self, stream=None):
+        if StreamTimer._hip is None:
+            lib()
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+            h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+            h.hipEventSynchronize.argtypes = [C.c_void_p]
+            h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+            h.hipEventDestroy.argtypes = [C.c_void_p]
+            StreamTimer._hip = h
+        self.stream = stream
+        self.ev = [C.c_void_p(), C.c_void_p()]
+        for e in self.ev:
+            if StreamTimer._hip.hipEventCreate(C.byref(e)) != 0:
+                raise OtError(OT_EHIP, "hipEventCreate failed")
+
+    def start(self):
+        if StreamTimer._hip.hipEventRecord(self.ev[0], self.stream) != 0:
+            raise OtError(OT_EHIP, "hipEventRecord failed")
+
+    def stop(self) -> float:
+        """Record the end event, wait for it and return the seconds between the two."""
+        h = StreamTimer._hip
+        if h.hipEventRecord(self.ev[1], self.stream) != 0 or h.hipEventSynchronize(self.ev[1]) != 0:
+            raise OtError(OT_EHIP, "hipEventRecord/hipEventSynchronize failed")
+        ms = C.c_float()
+        if h.hipEventElapsedTime(C.byref(ms), self.ev[0], self.ev[1]) != 0:
+            raise OtError(OT_EHIP, "hipEventElapsedTime failed")
+        return ms.value * 1e-3
+
+    def close(self):
+        for e in self.ev:
+            if e:
+                StreamTimer._hip.hipEventDestroy(e)
+        self.ev = [C.c_void_p(), C.c_void_p()]
 
 
 @dataclass
