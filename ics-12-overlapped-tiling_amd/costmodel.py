@@ -5,11 +5,11 @@
     python costmodel.py evaluate --data DIR --model FILE --report FILE
     python costmodel.py loso     --data DIR --report FILE
 
-collect  runs the exhaustive autotuner (ot_tune_exhaustive) over seeded random
-         problem shapes of every stencil (1D up to 2^24 points, 2D up to 4096^2,
-         3D up to 384^3) into DIR/timings.csv (resumable: rerunning continues
-         where it stopped) and records the wall time of every shape's sweep in
-         DIR/sweeps.json.
+collect  runs the exhaustive autotuner (ot_tune_exhaustive, in a child process
+         under a hang watchdog) over seeded random problem shapes of every
+         stencil (1D up to 2^24 points, 2D up to 4096^2, 3D up to 384^3) into
+         DIR/timings.csv (resumable: rerunning continues where it stopped) and
+         records the wall time of every shape's sweep in DIR/sweeps.json.
 train    fits the MLP (ot_model_*: HIP forward / backward / Adam kernels) on
          the shapes not held out (80 %, held out per stencil by shape).
 evaluate ranks every held-out shape's configuration space with the model and
@@ -77,7 +77,7 @@ def base_params(name, n3):
                              y=y, z=4, e=1)
 
 
-def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float, stencils=None):
+def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float, stencils=None, watchdog=15.0):
     """Exhaustive sweeps of the shapes drawn for `seed`, appended to
     out_dir/timings.csv. `stencils` (names) restricts the sweep to those
     stencils' shapes; the draw itself is unchanged, so a restricted run
@@ -90,9 +90,16 @@ def collect(out_dir: str, per_stencil: int, seed: int, time_limit: float, stenci
     for i, (name, n3) in enumerate(todo):
         key = shape_key(name, n3)
         t0 = time.perf_counter()
-        best, best_s, measured = ottune.exhaustive(base_params(name, n3), csv_path, warmup=1, reps=3,
-                                                   time_limit_s=time_limit)
+        before = ottune._data_rows(csv_path)
+        # each shape's sweep runs in a child process under the hang watchdog
+        best, best_s, info = ottune.exhaustive_supervised(base_params(name, n3), csv_path, warmup=1, reps=3,
+                                                          time_limit_s=time_limit, watchdog_s=watchdog,
+                                                          log=lambda m: print(m, flush=True))
+        measured = ottune._data_rows(csv_path) - before - len(info["hung"])
         wall = time.perf_counter() - t0
+        if best is None:
+            print(f"[{i + 1}/{len(todo)}] {key}: no configuration measured", flush=True)
+            continue
         if measured:
             sweeps[key] = {"wall_s": wall, "configs": measured}
             json.dump(sweeps, open(sweeps_path, "w"), indent=1)
@@ -271,6 +278,8 @@ def main():
     c.add_argument("--seed", type=int, default=2024)
     c.add_argument("--time-limit", type=float, default=0.05, help="watchdog: seconds per solve")
     c.add_argument("--stencils", default="", help="comma-separated stencil names to sweep (default: all)")
+    c.add_argument("--watchdog", type=float, default=15.0,
+                   help="seconds without a finished configuration before the sweep's process is restarted")
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--model", required=True)
@@ -291,7 +300,7 @@ def main():
     lo.add_argument("--seed", type=int, default=1)
     a = ap.parse_args()
     if a.cmd == "collect":
-        collect(a.out, a.per_stencil, a.seed, a.time_limit, [x for x in a.stencils.split(",") if x])
+        collect(a.out, a.per_stencil, a.seed, a.time_limit, [x for x in a.stencils.split(",") if x], a.watchdog)
     elif a.cmd == "train":
         train(a.data, a.model, a.epochs, a.batch, a.lr, a.seed)
     elif a.cmd == "evaluate":

@@ -30,8 +30,18 @@ struct LaunchGeom {
   float coef[4];
   int32_t wave_rows;     // footprint rows of the wave-per-tile kernel (2D, 3D), 0 = LDS kernel,
                          // -1 = 2D row-streaming kernel
-  int32_t reserved;
+  int32_t probe_sink;    // always 0: the timing probes store only when it is not, so their
+                         // work cannot be removed as dead code
+  ot_clock_record* clocks;  // per-wave clock records (OT_VARIANT_DUMP_CLOCKS), else null
 };
+
+// Clock-record slots per workgroup of the timing probes (1024 threads / 64).
+constexpr int kClockSlotsPerBlock = 16;
+
+// Variant bits that select the 2D timing-probe kernels (k2p.hip).
+constexpr int kProbeBits = OT_VARIANT_TIME_ONLY_GMEM | OT_VARIANT_TIME_ONLY_INSTR | OT_VARIANT_TIME_ONLY_SMEM |
+                           OT_VARIANT_DUMP_CLOCKS;
+
 
 // Register budget (3*T*rows per lane) of the wave-pipelined 3D kernels.
 constexpr int kW3RegBudget = 192;
@@ -58,6 +68,10 @@ struct FieldPtrs {
   float* dst[3];
   const float* ro[2];
 };
+
+// Timing-probe launches of the 2D LDS kernel (jacobi-2d, gradient-2d).
+ot_status launch_2d_probe(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
+                          void* stream, bool prepare);
 
 // Batched 2D launches (kbatch.hip): one image of the arena, and the arguments
 // of one persistent launch over the (image, tile) work queue.

@@ -58,6 +58,7 @@ void shape_2d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* 
 
 ot_status launch_2d(int stencil, int t_eff, int ept, const LaunchGeom& g, const FieldPtrs& f,
                     void* stream, bool prepare) {
+  if (g.variant & kProbeBits) return launch_2d_probe(stencil, t_eff, ept, g, f, stream, prepare);
   if (g.wave_rows < 0) {
     switch (stencil) {
       case OT_J2D: return launch_2ds_j2d(t_eff, g, f, stream, prepare);

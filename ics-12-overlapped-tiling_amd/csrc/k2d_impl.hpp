@@ -43,11 +43,24 @@ struct K2 {
   static constexpr int NPLANES = (ST == OT_FDTD2D || ST == OT_RICIAN2D) ? 3 : 2;
 };
 
+// Timing-probe modes of tile2d (PM): 0 the stencil itself; 1 the same with clock
+// stamps; 2 global-memory phases only; 3 arithmetic only; 4 on-chip phase only
+// (OT_VARIANT_TIME_ONLY_*, OT_VARIANT_DUMP_CLOCKS).
+enum { kProbeNone = 0, kProbeClocks = 1, kProbeGmem = 2, kProbeInstr = 3, kProbeSmem = 4 };
+
 // Advance output tile (bxi, byi) of the domain described by g by t_eff steps.
-// Called by the per-grid kernel below and by the batched work-queue kernel.
-template <int ST, int E>
+// Called by the per-grid kernel below, by the batched work-queue kernel and,
+// with PM != 0, by the timing-probe kernel, which passes clk[5] for the
+// s_memtime stamps P1..P3 (the caller takes P0 and P4).
+template <int ST, int E, int PM = kProbeNone>
 __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, int t_eff, int bxi, int byi,
-                                       float* lds) {
+                                       float* lds, uint64_t* clk = nullptr) {
+  static_assert(PM == kProbeNone || ST == OT_J2D || ST == OT_G2D,
+                "the timing probes exist for jacobi-2d and gradient-2d");
+  constexpr bool kGmem = PM == kProbeGmem, kInstr = PM == kProbeInstr, kSmem = PM == kProbeSmem;
+  auto stamp = [&](int k) __attribute__((always_inline)) {
+    if constexpr (PM != kProbeNone) clk[k] = __builtin_amdgcn_s_memtime();
+  };
   constexpr int R = K2<ST>::R;
   const int H = R * t_eff;
   const int FX = g.tile[0] + 2 * H;
@@ -69,10 +82,11 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
     gidx[i] = g.origin + (int64_t)gy * g.sy + gx;
   }
   if (g.phase_limit == 1) return;
+  stamp(1);
 
   float u[K2<ST>::NRW][E];
   float ro[K2<ST>::NRO > 0 ? K2<ST>::NRO : 1][E];
-  if (g.phase_limit != 3) {
+  if (g.phase_limit != 3 && !kSmem) {
 #pragma unroll
     for (int k = 0; k < K2<ST>::NRW; ++k)
 #pragma unroll
@@ -90,10 +104,24 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
 
   // LDS addressing helpers
   auto at = [&](int p, int y, int x) -> float& { return lds[p * plane + y * FX + x]; };
+  if constexpr (kGmem) {
+    // global-memory phases only: the loaded footprint goes straight back out
+    stamp(2);
+    stamp(3);
+    if (g.phase_limit != 0) return;
+    const bool wx = tx >= H && tx < H + g.tile[0];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int y = ty * E + i;
+      if (wx && y >= H && y < H + g.tile[1] && inside[i]) f.dst[0][gidx[i]] = u[0][i];
+    }
+    return;
+  }
 
   int cur = 0;
   // publish the initial state
-  if constexpr (ST == OT_FDTD2D) {
+  if constexpr (kInstr) {
+  } else if constexpr (ST == OT_FDTD2D) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       at(0, ty * E + i, tx) = u[0][i];
@@ -104,7 +132,8 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
 #pragma unroll
     for (int i = 0; i < E; ++i) at(0, ty * E + i, tx) = u[0][i];
   }
-  __syncthreads();
+  if constexpr (!kInstr) __syncthreads();
+  stamp(2);
 
   const int t_begin = (g.phase_limit == 3) ? 1 : 0;
   const int t_end = (g.phase_limit == 2) ? 1 : t_eff;
@@ -119,10 +148,11 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
         nu[i] = u[0][i];
         if (!edge_x && y >= 1 && y <= FY - 2) {
           const float c = u[0][i];
-          const float tc = (i > 0) ? u[0][i - 1] : at(cur, y - 1, tx);
-          const float bc = (i < E - 1) ? u[0][i + 1] : at(cur, y + 1, tx);
-          const float ml = at(cur, y, tx - 1);
-          const float mr = at(cur, y, tx + 1);
+          // arithmetic-only probe: every neighbour outside the strip is the centre
+          const float tc = (i > 0) ? u[0][i - 1] : (kInstr ? c : at(cur, y - 1, tx));
+          const float bc = (i < E - 1) ? u[0][i + 1] : (kInstr ? c : at(cur, y + 1, tx));
+          const float ml = kInstr ? c : at(cur, y, tx - 1);
+          const float mr = kInstr ? c : at(cur, y, tx + 1);
           float v;
           if constexpr (ST == OT_J2D) {
             v = j2d(ml, c, mr, tc, bc);
@@ -141,7 +171,7 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
       }
 #pragma unroll
       for (int i = 0; i < E; ++i) u[0][i] = nu[i];
-      if (!last) {
+      if (!last && !kInstr) {
 #pragma unroll
         for (int i = 0; i < E; ++i) at(nxt, ty * E + i, tx) = nu[i];
         __syncthreads();
@@ -246,7 +276,12 @@ __device__ __forceinline__ void tile2d(const LaunchGeom& g, const FieldPtrs& f, 
       }
     }
   }
+  stamp(3);
   if (g.phase_limit != 0) return;
+  // the arithmetic-only and on-chip-only probes store nothing (probe_sink is 0)
+  if constexpr (kInstr || kSmem) {
+    if (g.probe_sink == 0) return;
+  }
 
   // write back the valid interior of the tile
   const bool wx = tx >= H && tx < H + g.tile[0];

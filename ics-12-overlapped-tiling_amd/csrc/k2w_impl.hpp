@@ -255,7 +255,8 @@ __global__ void __launch_bounds__(kWaves * 64) k2w_batch(BArgs a, int /*t_eff*/,
     g.variant = a.variant;
     for (int i = 0; i < 4; ++i) g.coef[i] = a.coef[i];
     g.wave_rows = FR;
-    g.reserved = 0;
+    g.probe_sink = 0;
+    g.clocks = nullptr;
     tile<ST, T, FR>(g, a.base, tx, ty);
   }
 }

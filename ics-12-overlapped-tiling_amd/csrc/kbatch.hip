@@ -51,6 +51,9 @@ __global__ void __launch_bounds__(1024) k2d_batch(BArgs a, int t_eff, int launch
     g.phase_limit = a.phase_limit;
     g.variant = a.variant;
     for (int i = 0; i < 4; ++i) g.coef[i] = a.coef[i];
+    g.wave_rows = 0;
+    g.probe_sink = 0;
+    g.clocks = nullptr;
     tile2d<ST, E>(g, a.base, t_eff, tx, ty, lds);
     __syncthreads();  // LDS is reused by the next item
   }
@@ -274,6 +277,8 @@ ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_im
       p.elems_per_thread < 1)
     return set_error(OT_EINVAL, "invalid time tile / steps / block");
   if (p.phase_limit < 0 || p.phase_limit > 3) return set_error(OT_EINVAL, "phase limit must be 0..3");
+  if (p.variant & kProbeBits)
+    return set_error(OT_EUNSUPPORTED, "the timing probes run per grid (ot_plan_create), not in batches");
   const int32_t tile[3] = {p.block_x, p.block_y * p.elems_per_thread, 1};
   // items: every tile of every image, largest image first
   std::vector<int> order(n_images);

@@ -167,6 +167,9 @@ struct ot_plan {
   // per-launch events (OT_RUN_TIME_LAUNCHES)
   hipEvent_t* lev = nullptr;
   int nlev = 0;
+  // clock records of the last launch (OT_VARIANT_DUMP_CLOCKS)
+  ot_clock_record* clocks = nullptr;
+  int64_t nclocks = 0;
 };
 
 static void fill_ptrs(const ot_plan* pl, int launch, float* const* rw_a, float* const* rw_b,
@@ -182,6 +185,13 @@ static void fill_ptrs(const ot_plan* pl, int launch, float* const* rw_a, float* 
 
 // Enqueue the whole time loop. With lev (2 events per launch) every launch is
 // bracketed by its own event pair.
+// Clears the clock records before a probe launch (a kernel rather than
+// hipMemsetAsync, so that a captured graph replays it correctly).
+__global__ void zero_words(uint32_t* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 static ot_status enqueue_loop(ot_plan* pl, float* const* rw_a, float* const* rw_b, const float* const* ro,
                               hipStream_t s, hipEvent_t* lev) {
   const int T = pl->p.time_tile;
@@ -190,6 +200,13 @@ static ot_status enqueue_loop(ot_plan* pl, float* const* rw_a, float* const* rw_
     const int t_eff = (pl->p.time_steps - base) < T ? (pl->p.time_steps - base) : T;
     FieldPtrs f;
     fill_ptrs(pl, l, rw_a, rw_b, ro, f);
+    if (pl->clocks) {
+      // the records keep only the last launch's waves
+      const int64_t words = pl->nclocks * (int64_t)(sizeof(ot_clock_record) / 4);
+      hipLaunchKernelGGL(zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
+                         (uint32_t*)pl->clocks, words);
+      OT_HIP(hipGetLastError());
+    }
     if (lev) OT_HIP(hipEventRecord(lev[2 * l], s));
     ot_status st = launch_any(pl->p.stencil, pl->d.dims, t_eff, pl->p.elems_per_thread, pl->g, f, s, false);
     if (st != OT_OK) return st;
@@ -258,6 +275,11 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
                                        ? -1
                                        : k2w_rows(params->stencil, params->time_tile, d.tile))
                                 : 0;
+  if ((params->variant & kProbeBits) && d.dims != 2) {
+    delete pl;
+    return set_error(OT_EUNSUPPORTED,
+                     "--time-only and --dump-clocks exist for jacobi-2d and gradient-2d (as in the reference)");
+  }
   // Check the instantiations this plan will launch (full T and the shortened
   // last launch) exist, and raise their LDS limits now.
   FieldPtrs f;
@@ -269,11 +291,33 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   pl->d.registers = g_prepared_regs;
   if (st == OT_OK && last > 0)
     st = launch_any(params->stencil, d.dims, last, params->elems_per_thread, g, f, nullptr, true);
+  if (st == OT_OK && (params->variant & OT_VARIANT_DUMP_CLOCKS)) {
+    pl->nclocks = (int64_t)d.grid_blocks[0] * d.grid_blocks[1] * kClockSlotsPerBlock;
+    const size_t bytes = (size_t)pl->nclocks * sizeof(ot_clock_record);
+    hipError_t e = hipMalloc((void**)&pl->clocks, bytes);
+    if (e == hipSuccess) e = hipMemset(pl->clocks, 0, bytes);
+    if (e != hipSuccess) st = set_error(OT_EHIP, "clock records: %s", hipGetErrorString(e));
+    pl->g.clocks = pl->clocks;
+  }
   if (st != OT_OK) {
+    if (pl->clocks) (void)hipFree(pl->clocks);
     delete pl;
     return st;
   }
   *plan = pl;
+  return OT_OK;
+}
+
+ot_status ot_plan_clocks(ot_plan* plan, ot_clock_record* out, int32_t capacity, int32_t* count) {
+  if (!plan || !count || capacity < 0 || (capacity > 0 && !out)) return set_error(OT_EINVAL, "null argument");
+  if (!plan->clocks) return set_error(OT_EINVAL, "the plan was not created with OT_VARIANT_DUMP_CLOCKS");
+  *count = (int32_t)plan->nclocks;
+  const int64_t n = capacity < plan->nclocks ? capacity : plan->nclocks;
+  if (n > 0) {
+    // the plan's launches may have run on any stream
+    OT_HIP(hipDeviceSynchronize());
+    OT_HIP(hipMemcpy(out, plan->clocks, (size_t)n * sizeof(ot_clock_record), hipMemcpyDeviceToHost));
+  }
   return OT_OK;
 }
 
@@ -293,6 +337,7 @@ void ot_plan_destroy(ot_plan* plan) {
   if (plan->own_stream) (void)hipStreamDestroy(plan->own_stream);
   for (int i = 0; i < plan->nlev; ++i) (void)hipEventDestroy(plan->lev[i]);
   delete[] plan->lev;
+  if (plan->clocks) (void)hipFree(plan->clocks);
   delete plan;
 }
 

@@ -15,9 +15,18 @@
 //   * --verify compares the overlapped-tiled result with the non-tiled
 //     one-step-per-launch kernels (ot_naive_run) using the reference's L2 test
 //     (compareResults, jacobi-2d.cpp:517-551), and also prints the max relative
-//     error; --dump-clocks and --break-threads are accepted and have no effect.
+//     error;
+//   * --time-only gmem|instr|smem and -c/--dump-clocks run timing-probe
+//     instantiations of the 2D LDS kernel (jacobi-2d and gradient-2d, the
+//     drivers that have --time-only); the clock CSV goes to stderr as in the
+//     reference (jacobi-2d.cpp:957-997), one row per wave of one compute unit,
+//     stamps from s_memtime; --break-threads is accepted and has no effect;
+//   * a malformed option value is an error (usage, exit status 1).
 #include <algorithm>
 #include <chrono>
+#include <climits>
+#include <cstdint>
+#include <stdexcept>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -41,12 +50,38 @@ void printValue(const char* name, T value) {
 struct Options {
   std::string stencil;
   ot_params p{};
-  bool verify = false, help = false, graph = false;
+  bool verify = false, help = false, graph = false, clocks = false;
   std::string load, save, geometry = "reference", time_only;
   uint64_t seed = 123456;
 };
 
-bool parse_bool(const std::string& v) { return v == "1" || v == "true" || v == "on" || v == "yes"; }
+// Option values: the whole text must be the number (std::stoi alone accepts
+// "12abc"); anything else throws std::invalid_argument / std::out_of_range.
+int32_t to_int(const std::string& v) {
+  size_t pos = 0;
+  const long x = std::stol(v, &pos);
+  if (pos != v.size() || x < INT32_MIN || x > INT32_MAX) throw std::invalid_argument(v);
+  return (int32_t)x;
+}
+float to_float(const std::string& v) {
+  size_t pos = 0;
+  const float x = std::stof(v, &pos);
+  if (pos != v.size()) throw std::invalid_argument(v);
+  return x;
+}
+uint64_t to_u64(const std::string& v) {
+  size_t pos = 0;
+  if (!v.empty() && v[0] == '-') throw std::invalid_argument(v);
+  const unsigned long long x = std::stoull(v, &pos);
+  if (pos != v.size()) throw std::invalid_argument(v);
+  return (uint64_t)x;
+}
+// boost::program_options' bool syntax (the reference's -c arg)
+bool parse_bool(const std::string& v) {
+  if (v == "1" || v == "true" || v == "on" || v == "yes") return true;
+  if (v == "0" || v == "false" || v == "off" || v == "no") return false;
+  throw std::invalid_argument(v);
+}
 
 int usage(const char* prog) {
   std::cerr << "Usage: " << prog << " [options]\n"
@@ -63,9 +98,9 @@ int usage(const char* prog) {
                "  -f [ --load-kernel ] arg           Load plan description from disk\n"
                "  -w [ --save-kernel ] arg           Save plan description to disk\n"
                "  -v [ --verify ]                    Verify results\n"
-               "  -c [ --dump-clocks ] arg (=0)      Accepted for compatibility\n"
+               "  -c [ --dump-clocks ] arg (=0)      Dump per-wave phase clocks to stderr (2D LDS kernel)\n"
                "  -b [ --break-threads ]             Accepted for compatibility\n"
-               "  --time-only arg                    Not supported by this engine\n"
+               "  --time-only arg                    Time one phase only: gmem | instr | smem (2D LDS kernel)\n"
                "  --stencil arg                      Stencil name (default: from program name)\n"
                "  --geometry arg (=reference)        reference | tile\n"
                "  --nx / --ny / --nz arg             Per-axis extents (tile geometry)\n"
@@ -76,7 +111,18 @@ int usage(const char* prog) {
   return 1;
 }
 
+int parse_args(int argc, char** argv, Options& o);
+
 int parse(int argc, char** argv, Options& o) {
+  try {
+    return parse_args(argc, argv, o);
+  } catch (const std::exception& e) {
+    std::cerr << "invalid option value: " << e.what() << "\n";
+    return -1;
+  }
+}
+
+int parse_args(int argc, char** argv, Options& o) {
   std::map<std::string, std::string> alias = {
       {"-h", "help"}, {"-n", "problem-size"}, {"-t", "time-steps"}, {"-x", "block-size-x"},
       {"-y", "block-size-y"}, {"-z", "block-size-z"}, {"-e", "elements-per-thread"},
@@ -107,32 +153,32 @@ int parse(int argc, char** argv, Options& o) {
     }
     ot_params& p = o.p;
     if (key == "help") o.help = true;
-    else if (key == "problem-size") p.problem_size = std::stoi(val);
-    else if (key == "time-steps") p.time_steps = std::stoi(val);
-    else if (key == "block-size-x") p.block_x = std::stoi(val);
-    else if (key == "block-size-y") p.block_y = std::stoi(val);
-    else if (key == "block-size-z") p.block_z = std::stoi(val);
-    else if (key == "elements-per-thread") p.elems_per_thread = std::stoi(val);
-    else if (key == "time-tile-size") p.time_tile = std::stoi(val);
-    else if (key == "phase-limit") p.phase_limit = std::stoi(val);
+    else if (key == "problem-size") p.problem_size = to_int(val);
+    else if (key == "time-steps") p.time_steps = to_int(val);
+    else if (key == "block-size-x") p.block_x = to_int(val);
+    else if (key == "block-size-y") p.block_y = to_int(val);
+    else if (key == "block-size-z") p.block_z = to_int(val);
+    else if (key == "elements-per-thread") p.elems_per_thread = to_int(val);
+    else if (key == "time-tile-size") p.time_tile = to_int(val);
+    else if (key == "phase-limit") p.phase_limit = to_int(val);
     else if (key == "load-kernel") o.load = val;
     else if (key == "save-kernel") o.save = val;
     else if (key == "verify") o.verify = true;
-    else if (key == "dump-clocks") (void)parse_bool(val);
+    else if (key == "dump-clocks") o.clocks = parse_bool(val);
     else if (key == "break-threads") {
     } else if (key == "time-only") o.time_only = val;
     else if (key == "stencil") o.stencil = val;
     else if (key == "geometry") o.geometry = val;
-    else if (key == "nx") p.nx = std::stoi(val);
-    else if (key == "ny") p.ny = std::stoi(val);
-    else if (key == "nz") p.nz = std::stoi(val);
+    else if (key == "nx") p.nx = to_int(val);
+    else if (key == "ny") p.ny = to_int(val);
+    else if (key == "nz") p.nz = to_int(val);
     else if (key == "graph") o.graph = true;
-    else if (key == "alpha") p.coef[0] = std::stof(val);
-    else if (key == "sigma2") p.coef[0] = std::stof(val);
-    else if (key == "lambda") p.coef[1] = std::stof(val);
-    else if (key == "dt") p.coef[2] = std::stof(val);
+    else if (key == "alpha") p.coef[0] = to_float(val);
+    else if (key == "sigma2") p.coef[0] = to_float(val);
+    else if (key == "lambda") p.coef[1] = to_float(val);
+    else if (key == "dt") p.coef[2] = to_float(val);
     else if (key == "fdtd-physical") p.variant |= OT_VARIANT_FDTD_PHYSICAL;
-    else if (key == "seed") o.seed = std::stoull(val);
+    else if (key == "seed") o.seed = to_u64(val);
     else {
       std::cerr << "unrecognised option --" << key << "\n";
       return -1;
@@ -205,9 +251,16 @@ int main(int argc, char** argv) {
   if (parse(argc, argv, o) != 0) return usage(prog.c_str());
   if (o.help) return usage(prog.c_str());
   if (!o.time_only.empty()) {
-    std::cout << "ERROR: --time-only " << o.time_only << " is not supported by this engine\n";
-    return 1;
+    // jacobi-2d.cpp:622-629
+    if (o.time_only == "gmem") o.p.variant |= OT_VARIANT_TIME_ONLY_GMEM;
+    else if (o.time_only == "instr") o.p.variant |= OT_VARIANT_TIME_ONLY_INSTR;
+    else if (o.time_only == "smem") o.p.variant |= OT_VARIANT_TIME_ONLY_SMEM;
+    else {
+      std::cerr << "--time-only takes gmem, instr or smem (got '" << o.time_only << "')\n";
+      return usage(prog.c_str());
+    }
   }
+  if (o.clocks) o.p.variant |= OT_VARIANT_DUMP_CLOCKS;
   if (!o.load.empty()) {
     std::string st = o.stencil;
     if (!load_plan(o.load, st, o.p)) {
@@ -310,8 +363,28 @@ int main(int argc, char** argv) {
   printValue("Device GFlop/s", dev_pts * info.flops_per_point * o.p.time_steps / tm.wall_s / 1e9);
   printValue("GStencil/s", pts * o.p.time_steps / tm.event_s / 1e9);
 
+  if (o.clocks) {
+    // ClockData rows of one compute unit (the reference keeps SM 1,
+    // jacobi-2d.cpp:490-507, :957-997): here CU 1 of shader engine 0 on die 0;
+    // clock0..clock4 are s_memtime at P0..P4
+    int32_t n = 0;
+    if (ot_plan_clocks(plan, nullptr, 0, &n) != OT_OK) return fail("clocks");
+    std::vector<ot_clock_record> rec(n);
+    if (n > 0 && ot_plan_clocks(plan, rec.data(), n, &n) != OT_OK) return fail("clocks");
+    std::cerr << "bx,by,warp,sm,clock0,clock1,clock2,clock3,clock4,\n";
+    for (const ot_clock_record& r : rec) {
+      const unsigned cu = (r.hw_id >> 8) & 0xF, sh = (r.hw_id >> 12) & 1, se = (r.hw_id >> 13) & 0x7;
+      if (!r.valid || cu != 1 || sh != 0 || se != 0 || r.xcc_id != 0) continue;
+      std::cerr << r.block_x << "," << r.block_y << "," << r.wave << "," << cu << "," << r.clock[0] << ","
+                << r.clock[1] << "," << r.clock[2] << "," << r.clock[3] << "," << r.clock[4] << ",\n";
+    }
+  }
+
   int rc = 0;
-  if (o.verify && o.p.phase_limit == 0) {
+  if (o.verify && (o.p.variant & (OT_VARIANT_TIME_ONLY_GMEM | OT_VARIANT_TIME_ONLY_INSTR |
+                                  OT_VARIANT_TIME_ONLY_SMEM))) {
+    printValue("Correctness", "SKIPPED (--time-only results are not the stencil's)");
+  } else if (o.verify && o.p.phase_limit == 0) {
     std::vector<std::vector<float>> dev(d.num_rw, std::vector<float>(elems)), ref(d.num_rw, std::vector<float>(elems));
     for (int k = 0; k < d.num_rw; ++k) ot_memcpy_d2h(dev[k].data(), tm.final_buffer ? b[k] : a[k], bytes);
     std::vector<void*> na(d.num_rw), nb(d.num_rw);

@@ -24,6 +24,7 @@ STENCILS = [
 (J1D3, J1D5, J1D7, J2D, J3D, P2D, P3D27, P3D7, FDTD2D, G2D, G3D, RICIAN2D, TV2D, CS3D) = range(14)
 GEOM_TILE, GEOM_REFERENCE = 0, 1
 VARIANT_FDTD_PHYSICAL = 1
+VARIANT_TIME_ONLY_GMEM, VARIANT_TIME_ONLY_INSTR, VARIANT_TIME_ONLY_SMEM, VARIANT_DUMP_CLOCKS = 2, 4, 8, 16
 
 OT_OK, OT_EINVAL, OT_ERESOURCE, OT_EHIP, OT_EUNSUPPORTED = range(5)
 
@@ -112,6 +113,7 @@ SIGNATURES = [
     ("ot_plan_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ot_run", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_void_p, C.c_int32, C.POINTER(Timing)]),
     ("ot_launch", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_int32, C.c_void_p]),
+    ("ot_plan_clocks", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     ("ot_naive_run", C.c_int, [C.POINTER(Params), _PP, _PP, _PP, C.c_void_p, C.POINTER(Timing)]),
     ("ot_batch_layout", C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(Image),
                                   C.POINTER(C.c_int64)]),

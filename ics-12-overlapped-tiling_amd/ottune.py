@@ -12,12 +12,19 @@ scripts/parse-weka-results.py:28-35). This module exposes the same steps:
   analytic_seconds(...) ot_model_analytic (run-model.py restated, MI355X constants)
   measure(...)          ot_tune_measure (median EventElapsed of k solves)
   exhaustive(...)       ot_tune_exhaustive (resumable CSV with the driver keys)
+  exhaustive_supervised the same sweep in a child process under a hang watchdog
   CostModel             the MLP (HIP training / inference kernels)
   read_csv(path)        rows of a results file as dicts
 """
 import csv
 import ctypes as C
+import json
 import math
+import os
+import signal
+import subprocess
+import sys
+import time
 
 import numpy as np
 
@@ -73,6 +80,120 @@ def exhaustive(p: Params, csv_path: str, warmup=1, reps=3, time_limit_s=0.0):
     check(lib().ot_tune_exhaustive(C.byref(p), csv_path.encode(), warmup, reps, time_limit_s, C.byref(best),
                                    C.byref(bs), C.byref(m)))
     return best, bs.value, m.value
+
+
+CSV_COLUMNS = ["Stencil", "Dimensions", "Nx", "Ny", "Nz", "Time Steps", "Geometry", "Time Tile Size",
+               "Block Size X", "Block Size Y", "Block Size Z", "Elements/Thread", "Num Blocks X", "Num Blocks Y",
+               "Num Blocks Z", "Threads", "LDS Bytes", "Shared Size", "Register Usage", "Wave Kernel",
+               "EventElapsed", "Status"]
+
+
+def config_key(p: Params):
+    """The identity of one configuration in a results file: stencil, extents,
+    steps, geometry and tile (the columns ot_tune_exhaustive resumes on)."""
+    d = otlib.derive(p)
+    return (otlib.STENCILS[p.stencil], d.real_size[0], d.real_size[1], d.real_size[2], p.time_steps, p.geometry,
+            p.time_tile, p.block_x, p.block_y, p.block_z, p.elems_per_thread)
+
+
+def _row_key(r):
+    return (r["Stencil"],) + tuple(int(r[c]) for c in ("Nx", "Ny", "Nz", "Time Steps", "Geometry", "Time Tile Size",
+                                                       "Block Size X", "Block Size Y", "Block Size Z",
+                                                       "Elements/Thread"))
+
+
+def _data_rows(path: str) -> int:
+    try:
+        with open(path) as f:
+            return max(0, sum(1 for _ in f) - 1)
+    except OSError:
+        return 0
+
+
+def _append_status_row(path: str, p: Params, status: str) -> None:
+    """Record configuration p with no measurement (EventElapsed -1) and the
+    given Status, so that a resumed sweep skips it."""
+    d = otlib.derive(p)
+    new = not os.path.exists(path) or os.path.getsize(path) == 0
+    with open(path, "a") as f:
+        if new:
+            f.write(",".join(CSV_COLUMNS) + "\n")
+        vals = [otlib.STENCILS[p.stencil], d.dims, d.real_size[0], d.real_size[1], d.real_size[2], p.time_steps,
+                p.geometry, p.time_tile, p.block_x, p.block_y, p.block_z, p.elems_per_thread, d.grid_blocks[0],
+                d.grid_blocks[1], d.grid_blocks[2], d.threads_per_block, d.lds_bytes, d.shared_size, 0,
+                int(features(p)[23]), -1, status]
+        f.write(",".join(str(v) for v in vals) + "\n")
+
+
+def _params_json(p: Params) -> str:
+    return json.dumps({n: (list(getattr(p, n)) if n == "coef" else getattr(p, n)) for n, _ in Params._fields_})
+
+
+def _params_from_json(text: str) -> Params:
+    p = Params()
+    for n, v in json.loads(text).items():
+        if n == "coef":
+            for i, c in enumerate(v):
+                p.coef[i] = c
+        else:
+            setattr(p, n, v)
+    return p
+
+
+def exhaustive_supervised(p: Params, csv_path: str, warmup=1, reps=3, time_limit_s=0.0, watchdog_s=15.0,
+                          startup_s=300.0, max_restarts=50, child_cmd=None, log=None):
+    """ot_tune_exhaustive in a child process under a hang watchdog, as the
+    reference runs every configuration in its own process and terminates one
+    that runs past 15 s (scripts/utils.py:130-145). The sweep writes one row
+    per configuration; when the file gains no row for watchdog_s seconds
+    (startup_s for the child's first row), the child's whole process group is
+    killed, the configuration it was on (the first of the space missing from
+    the file) is recorded with Status "hang" ("crash" when the child died by
+    itself) and a fresh child resumes the sweep. This process only reads the
+    file and enumerates the space on the host, so it never touches the GPU.
+    child_cmd replaces the child's command line (a sweep stand-in for tests).
+    Returns (best Params or None, best seconds, {"restarts", "hung": [keys]})."""
+    sp = space(p)
+    keys = [config_key(q) for q in sp]
+    cmd = child_cmd or [sys.executable, os.path.abspath(__file__), "sweep-child", _params_json(p), csv_path,
+                        str(warmup), str(reps), repr(float(time_limit_s))]
+    hung, restarts = [], 0
+    while True:
+        proc = subprocess.Popen(cmd, start_new_session=True, stdout=subprocess.DEVNULL)
+        rows, last = _data_rows(csv_path), time.monotonic()
+        limit, killed = startup_s, False
+        while proc.poll() is None:
+            time.sleep(0.05)
+            n = _data_rows(csv_path)
+            if n != rows:
+                rows, last, limit = n, time.monotonic(), watchdog_s
+            elif time.monotonic() - last > limit:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.wait()
+                killed = True
+                break
+        if proc.returncode == 0 and not killed:
+            break
+        done = {_row_key(r) for r in read_csv(csv_path)} if os.path.exists(csv_path) else set()
+        nxt = next((q for q, k in zip(sp, keys) if k not in done), None)
+        if nxt is None:
+            break
+        status = "hang" if killed else "crash"
+        _append_status_row(csv_path, nxt, status)
+        hung.append(config_key(nxt))
+        if log:
+            log(f"watchdog: {status} at T={nxt.time_tile} {nxt.block_x}x{nxt.block_y}x{nxt.block_z} "
+                f"e{nxt.elems_per_thread}; restarting the sweep")
+        restarts += 1
+        if restarts > max_restarts:
+            raise otlib.OtError(otlib.OT_EHIP, f"sweep restarted {restarts} times without finishing")
+    best, best_s = None, float("inf")
+    want = set(keys)
+    for r in read_csv(csv_path):
+        if r["Status"] == "ok" and _row_key(r) in want and float(r["EventElapsed"]) < best_s:
+            best_s = float(r["EventElapsed"])
+            best = next(q for q, k in zip(sp, keys) if k == _row_key(r))
+    return best, best_s, {"restarts": restarts, "hung": hung}
 
 
 def read_csv(path: str):
@@ -181,3 +302,9 @@ class CostModel:
             self.close()
         except Exception:
             pass
+
+
+if __name__ == "__main__" and len(sys.argv) == 7 and sys.argv[1] == "sweep-child":
+    # the child of exhaustive_supervised: one resumable exhaustive sweep
+    _p = _params_from_json(sys.argv[2])
+    exhaustive(_p, sys.argv[3], warmup=int(sys.argv[4]), reps=int(sys.argv[5]), time_limit_s=float(sys.argv[6]))

@@ -73,8 +73,25 @@ enum {
 
 /* variant flags (ot_params.variant) */
 enum {
-  OT_VARIANT_FDTD_PHYSICAL = 1  /* Hz uses -Ey(y) (curl sign); default keeps the
-                                   reference's "+eyCenter" (fdtd-2d.cpp:320-322) */
+  OT_VARIANT_FDTD_PHYSICAL = 1,  /* Hz uses -Ey(y) (curl sign); default keeps the
+                                    reference's "+eyCenter" (fdtd-2d.cpp:320-322) */
+  /* Timing probes of the 2D LDS kernel (jacobi-2d and gradient-2d, the drivers
+   * with --time-only, jacobi-2d.cpp:258-266, :296-341, :368-433, :462-481).
+   * Results are NOT the stencil's: the modes time one part of the kernel.
+   *   GMEM:  the global-memory phases only: every footprint point is loaded and
+   *          the tile interior stored back, no step is computed, no LDS used;
+   *   INSTR: the arithmetic only: one load per point, every step computed with
+   *          the neighbours taken from the thread's own registers, no LDS, no
+   *          barrier and no store;
+   *   SMEM:  the on-chip phase only: no load (each point starts at its x
+   *          index), every step through LDS with its barriers, no store. */
+  OT_VARIANT_TIME_ONLY_GMEM = 2,
+  OT_VARIANT_TIME_ONLY_INSTR = 4,
+  OT_VARIANT_TIME_ONLY_SMEM = 8,
+  /* Per-wave clock stamps at the kernel's phase boundaries (-c/--dump-clocks,
+   * jacobi-2d.cpp:202-205, :453-509, :957-997), read with ot_plan_clocks. 2D
+   * LDS kernel of jacobi-2d and gradient-2d; results are unchanged. */
+  OT_VARIANT_DUMP_CLOCKS = 16
 };
 
 typedef struct ot_stencil_info {
@@ -217,6 +234,28 @@ ot_status ot_run(ot_plan* plan, float* const* rw_a, float* const* rw_b,
  * baseTime) of the reference). Asynchronous on stream. */
 ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_dst,
                     const float* const* ro, int32_t base_time, void* stream);
+
+/* Clock stamps of one wavefront of the last launch of a plan created with
+ * OT_VARIANT_DUMP_CLOCKS (the reference's ClockData, jacobi-2d.cpp:854-858):
+ * s_memtime (shader clock) at P0 kernel entry, P1 after the index set-up, P2
+ * after the global loads and the LDS publish, P3 after the on-chip time steps
+ * and P4 after the stores, plus s_memrealtime (constant 100 MHz) at P0 and P4
+ * and the hardware ids of the wave's slot. */
+typedef struct ot_clock_record {
+  int32_t block_x, block_y; /* output tile of the workgroup */
+  int32_t wave;             /* wavefront index inside the workgroup */
+  uint32_t hw_id;           /* HW_ID register: wave 3:0, SIMD 5:4, CU 11:8, SH 12, SE 15:13 */
+  uint32_t xcc_id;          /* XCC_ID register: the accelerator complex die */
+  uint32_t valid;           /* 1 when the wave wrote this record */
+  uint64_t clock[5];
+  uint64_t realtime[2];
+} ot_clock_record;
+
+/* Copy the clock records of the plan's last launch to host memory: count
+ * receives the number of record slots (16 per workgroup, slot 16 * workgroup +
+ * wave), out gets min(capacity, count) of them (out may be NULL with capacity
+ * 0). Slots whose valid is 0 had no wavefront in the last launch. */
+ot_status ot_plan_clocks(ot_plan* plan, ot_clock_record* out, int32_t capacity, int32_t* count);
 
 /* Run the same problem with straightforward one-step-per-launch kernels (no
  * tiling, every tap from global memory): the counterpart of the reference CUDA

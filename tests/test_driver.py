@@ -34,6 +34,22 @@ def test_driver_rejects_unknown_option():
     assert rc == 1
 
 
+@pytest.mark.parametrize("args", [["-n", "abc"], ["-n", "12abc"], ["-t", ""], ["-s", "3.5"], ["-x", "99999999999"],
+                                  ["--alpha", "fast"], ["--seed", "-1"], ["-c", "maybe"],
+                                  ["--time-only", "gpu"]])
+def test_driver_malformed_value_prints_usage_and_exits_1(args):
+    rc, keys, r = run_driver("jacobi-2d", *args)
+    assert rc == 1 and "Usage:" in r.stderr and not keys
+
+
+def test_time_only_and_dump_clocks_exist_only_for_the_2d_lds_drivers():
+    # refused before any device work: the 3D drivers have no timing probes
+    rc, keys, r = run_driver("jacobi-3d", "-n", 32, "--time-only", "gmem")
+    assert rc == 1 and "jacobi-2d and gradient-2d" in r.stdout
+    rc, keys, r = run_driver("jacobi-3d", "-n", 32, "-c", "1")
+    assert rc == 1 and "jacobi-2d and gradient-2d" in r.stdout
+
+
 def test_every_stencil_has_a_driver(otlib):
     for name in otlib.STENCILS:
         assert os.path.exists(os.path.join(BIN, f"ot-{name}")), name
@@ -95,3 +111,30 @@ def test_save_and_load_plan(tmp_path):
     rc, k2, _ = run_driver("jacobi-2d", "-f", plan, "-v")
     assert rc == 0 and k2["Correctness"] == "PASSED"
     assert k1["Num Blocks X"] == k2["Num Blocks X"] and k1["Time Tile Size"] == k2["Time Tile Size"]
+
+
+def _clock_rows(stderr):
+    lines = stderr.splitlines()
+    i = lines.index("bx,by,warp,sm,clock0,clock1,clock2,clock3,clock4,")
+    return [[int(v) for v in ln.rstrip(",").split(",")] for ln in lines[i + 1:] if ln.count(",") == 9]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["jacobi-2d", "gradient-2d"])
+def test_driver_dump_clocks_and_time_only(name):
+    """-c 1 prints the reference's ClockData CSV (jacobi-2d.cpp:957-997) for the
+    waves of one compute unit, with stamps in phase order, and leaves the
+    result unchanged (verify passes); every --time-only mode runs to the end."""
+    base = ["-n", 700, "-t", 12, "-s", 4, "-x", 32, "-y", 8, "-e", 2]
+    rc, keys, r = run_driver(name, *base, "-c", 1, "-v")
+    assert rc == 0, r.stdout + r.stderr
+    assert keys["Correctness"] == "PASSED"
+    rows = _clock_rows(r.stderr)
+    assert rows and all(row[3] == 1 for row in rows)
+    for row in rows:
+        c = row[4:]
+        assert c[0] <= c[1] <= c[2] <= c[3] <= c[4], row
+    for mode in ("gmem", "instr", "smem"):
+        rc, keys, r = run_driver(name, *base, "--time-only", mode, "-v")
+        assert rc == 0, (mode, r.stdout + r.stderr)
+        assert float(keys["EventElapsed"]) > 0 and keys["Correctness"].startswith("SKIPPED")

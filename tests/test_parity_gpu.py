@@ -249,6 +249,11 @@ def test_enqueue_only_solves_match_synchronous(otlib):
 FULL_SIZE = [
     # (stencil, extents, steps, tiled (T, block, ept), T=1 (block, ept)): the bench workloads
     (otlib.J3D, (256, 256, 256), 32, (3, (58, 6, 19), 1), ((64, 4, 32), 4)),
+    # the deepest time tiles of the 256^3 sweep: the wave kernel at T=5 and the
+    # LDS kernel at T=8, down to its smallest tile 8 x 8 x 4
+    (otlib.J3D, (256, 256, 256), 32, (5, (54, 2, 20), 1), ((64, 4, 32), 4)),
+    (otlib.J3D, (256, 256, 256), 32, (8, (16, 8, 16), 1), ((64, 4, 32), 4)),
+    (otlib.J3D, (256, 256, 256), 32, (8, (8, 8, 4), 1), ((64, 4, 32), 4)),
     (otlib.G3D, (256, 256, 256), 16, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
     (otlib.P3D7, (256, 256, 256), 32, (2, (60, 4, 29), 1), ((64, 4, 32), 4)),
     (otlib.FDTD2D, (2048, 2048, 1), 100, (4, (64, 8, 1), 2), ((64, 4, 1), 4)),
@@ -261,7 +266,8 @@ FULL_SIZE = [
 ]
 
 
-@pytest.mark.parametrize("case", FULL_SIZE, ids=[f"{otlib.STENCILS[c[0]]}-full-T{c[3][0]}" for c in FULL_SIZE])
+@pytest.mark.parametrize("case", FULL_SIZE, ids=[f"{otlib.STENCILS[c[0]]}-full-T{c[3][0]}-{c[3][1][0]}x{c[3][1][1]}x{c[3][1][2]}"
+                                             for c in FULL_SIZE])
 def test_full_size_time_tiling_is_bitwise_t1(otlib, case):
     """At the measured workload sizes the time-tiled solve equals the T=1 solve
     bit for bit (size-independent property: same operations in the same order),

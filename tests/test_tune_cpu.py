@@ -142,3 +142,67 @@ def test_space_holds_all_2d_kernel_families(otlib):
                 assert d.lds_bytes == 0 and d.threads_per_block == 256, (name, q.time_tile, q.block_y)
         if name == "rician-denoise-2d":
             assert not [k for k in wave if k[2] >= 64 and k[0] > 5]
+
+
+_FAKE_SWEEP = r'''
+import csv, json, os, sys, time
+rows = json.load(open(sys.argv[1]))
+path, hang_at, crash_at = sys.argv[2], int(sys.argv[3]), int(sys.argv[4])
+cols = ["Stencil", "Nx", "Ny", "Nz", "Time Steps", "Geometry", "Time Tile Size", "Block Size X",
+        "Block Size Y", "Block Size Z", "Elements/Thread"]
+done = set()
+if os.path.exists(path):
+    with open(path) as f:
+        done = {tuple(r[c] for c in cols) for r in csv.DictReader(f)}
+for i, r in enumerate(rows):
+    if tuple(str(r[c]) for c in cols) in done:
+        continue
+    if i == hang_at:
+        time.sleep(600)   # a kernel that never returns
+    if i == crash_at:
+        sys.exit(3)       # a process that dies on this configuration
+    new = not os.path.exists(path)
+    with open(path, "a") as f:
+        if new:
+            f.write(",".join(r) + "\n")
+        f.write(",".join(str(v) for v in r.values()) + "\n")
+'''
+
+
+def test_supervised_sweep_kills_a_hung_child_and_resumes(otlib, tmp_path):
+    """The exhaustive sweep's watchdog (the reference's 15 s per-run watchdog,
+    scripts/utils.py:130-145), exercised on the CPU with a stand-in for the
+    sweep child: configuration 7 never returns and configuration 20 kills its
+    process. Each is recorded once with Status hang / crash, a fresh child
+    resumes, and every other configuration is measured exactly once."""
+    p = _p("jacobi-1d", (4096, 1, 1), 8, 1, 64, 1, 1, 1)
+    sp = ottune.space(p)
+    rows = []
+    for i, q in enumerate(sp):
+        k = ottune.config_key(q)
+        r = dict(zip(["Stencil", "Nx", "Ny", "Nz", "Time Steps", "Geometry", "Time Tile Size", "Block Size X",
+                      "Block Size Y", "Block Size Z", "Elements/Thread"], k))
+        full = {c: r.get(c, 0) for c in ottune.CSV_COLUMNS}
+        full["EventElapsed"] = 1e-3 + 1e-6 * ((i * 37) % 101)
+        full["Status"] = "ok"
+        rows.append(full)
+    jf = tmp_path / "rows.json"
+    jf.write_text(__import__("json").dumps(rows))
+    child = tmp_path / "fake_sweep.py"
+    child.write_text(_FAKE_SWEEP)
+    csv_path = str(tmp_path / "sweep.csv")
+    import sys
+    cmd = [sys.executable, str(child), str(jf), csv_path, "7", "20"]
+    msgs = []
+    best, best_s, info = ottune.exhaustive_supervised(p, csv_path, watchdog_s=0.5, startup_s=10.0, child_cmd=cmd,
+                                                      log=msgs.append)
+    assert info["restarts"] == 2 and len(msgs) == 2
+    assert info["hung"] == [ottune.config_key(sp[7]), ottune.config_key(sp[20])]
+    got = ottune.read_csv(csv_path)
+    assert len(got) == len(sp)
+    status = {ottune._row_key(r): r["Status"] for r in got}
+    assert status[ottune.config_key(sp[7])] == "hang" and status[ottune.config_key(sp[20])] == "crash"
+    assert sum(v == "ok" for v in status.values()) == len(sp) - 2
+    ok = [(float(r["EventElapsed"]), i) for i, r in enumerate(rows) if i not in (7, 20)]
+    assert best_s == pytest.approx(min(ok)[0])
+    assert ottune.config_key(best) == ottune.config_key(sp[min(ok)[1]])
