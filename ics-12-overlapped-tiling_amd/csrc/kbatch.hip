@@ -15,7 +15,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -216,34 +218,146 @@ static ot_status per_image_enqueue(ot_batch* b, float* const* rw_a, float* const
   return OT_OK;
 }
 
+// The persistent work-queue launches of one batch solve (counters already
+// cleared): launch l advances every item by min(T, steps - l T) steps.
+static ot_status enqueue_queue(const ot_params& p, int launches, int n_items, const LaunchShape& full,
+                               const LaunchShape& last, const BImage* images, const int32_t* items,
+                               int32_t* counters, float* const* rw_a, float* const* rw_b, const float* const* ro,
+                               hipStream_t s) {
+  const int T = p.time_tile;
+  const int num_rw = kTraits[p.stencil].num_rw, num_ro = kTraits[p.stencil].num_ro;
+  for (int l = 0; l < launches; ++l) {
+    const int t_eff = std::min(T, p.time_steps - l * T);
+    const LaunchShape& sh = (t_eff == T) ? full : last;
+    BArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.images = images;
+    a.items = items;
+    a.counters = counters;
+    a.n_items = n_items;
+    a.tile[0] = p.block_x;
+    a.tile[1] = p.block_y * p.elems_per_thread;
+    a.phase_limit = p.phase_limit;
+    a.variant = p.variant;
+    for (int i = 0; i < 4; ++i) a.coef[i] = p.coef[i];
+    const bool even = (l % 2) == 0;
+    for (int k = 0; k < num_rw; ++k) {
+      a.base.src[k] = even ? rw_a[k] : rw_b[k];
+      a.base.dst[k] = even ? rw_b[k] : rw_a[k];
+    }
+    for (int k = 0; k < num_ro; ++k) a.base.ro[k] = ro[k];
+    hipLaunchKernelGGL(sh.fn, dim3(sh.wgs), dim3(sh.bx, sh.by), sh.lds, s, a, t_eff, l);
+    OT_HIP(hipGetLastError());
+  }
+  return OT_OK;
+}
+
 static ot_status batch_enqueue(ot_batch* b, float* const* rw_a, float* const* rw_b, const float* const* ro,
                                hipStream_t s) {
   if (b->per_image_mode) return per_image_enqueue(b, rw_a, rw_b, ro, s);
   hipLaunchKernelGGL(zero_counters, dim3((b->launches + 255) / 256), dim3(256), 0, s, b->d_counters, b->launches);
   OT_HIP(hipGetLastError());
-  const int T = b->p.time_tile;
-  for (int l = 0; l < b->launches; ++l) {
-    const int t_eff = std::min(T, b->p.time_steps - l * T);
-    const LaunchShape& sh = (t_eff == T) ? b->full : b->last;
-    BArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.images = b->d_images;
-    a.items = b->d_items;
-    a.counters = b->d_counters;
-    a.n_items = b->n_items;
-    a.tile[0] = b->p.block_x;
-    a.tile[1] = b->p.block_y * b->p.elems_per_thread;
-    a.phase_limit = b->p.phase_limit;
-    a.variant = b->p.variant;
-    for (int i = 0; i < 4; ++i) a.coef[i] = b->p.coef[i];
-    const bool even = (l % 2) == 0;
-    for (int k = 0; k < b->num_rw; ++k) {
-      a.base.src[k] = even ? rw_a[k] : rw_b[k];
-      a.base.dst[k] = even ? rw_b[k] : rw_a[k];
+  return enqueue_queue(b->p, b->launches, b->n_items, b->full, b->last, b->d_images, b->d_items, b->d_counters,
+                       rw_a, rw_b, ro, s);
+}
+
+// ---- the allocation-free entry point ---------------------------------------
+// Its work queue (launch counters, image table, items) lives in `scratch`, a
+// file-scope device array of fixed size that every GPU the library runs on
+// holds (its own copy) until the process ends. Host code never passes the
+// array's name to the device: it takes each GPU's address once with
+// hipGetSymbolAddress and uses that.
+namespace {
+constexpr size_t kStaticScratchBytes = size_t(8) << 20;  // 8 MiB per GPU
+constexpr int kStaticMaxImages = 4096;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (with the device current) and kept.
+ot_status static_scratch(void** out) {
+  int dev = 0;
+  OT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(OT_EUNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(OT_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return OT_OK;
+}
+
+// One workgroup builds the work queue of ot_batch_create in device memory:
+// clears the launch counters, ranks the images largest first (ties by index,
+// the stable order of ot_batch_create), and writes the items of each image
+// (tile rows outer, tile columns inner) after those of the larger ones.
+__global__ void __launch_bounds__(1024) build_queue(const BImage* images, int n, int tile_x, int tile_y,
+                                                    int32_t* items, int n_items, int32_t* counters, int launches) {
+  __shared__ int s_img[kStaticMaxImages];     // image at rank r
+  __shared__ int s_off[kStaticMaxImages + 1]; // first item of rank r
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int i = tid; i < launches; i += nt) counters[i] = 0;
+  for (int i = tid; i < n; i += nt) {
+    const int64_t ai = (int64_t)images[i].w * images[i].h;
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const int64_t aj = (int64_t)images[j].w * images[j].h;
+      r += (aj > ai || (aj == ai && j < i)) ? 1 : 0;
     }
-    for (int k = 0; k < b->num_ro; ++k) a.base.ro[k] = ro[k];
-    hipLaunchKernelGGL(sh.fn, dim3(sh.wgs), dim3(sh.bx, sh.by), sh.lds, s, a, t_eff, l);
-    OT_HIP(hipGetLastError());
+    s_img[r] = i;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int r = 0; r < n; ++r) {
+      const BImage im = images[s_img[r]];
+      s_off[r] = acc;
+      acc += ((im.w + tile_x - 1) / tile_x) * ((im.h + tile_y - 1) / tile_y);
+    }
+    s_off[n] = acc;
+  }
+  __syncthreads();
+  for (int it = tid; it < n_items; it += nt) {
+    int lo = 0, hi = n - 1;  // the rank whose items hold `it`: s_off[lo] <= it < s_off[lo + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_off[mid] <= it) lo = mid;
+      else hi = mid - 1;
+    }
+    const int img = s_img[lo], k = it - s_off[lo];
+    const int tiles_x = (images[img].w + tile_x - 1) / tile_x;
+    items[3 * it] = img;
+    items[3 * it + 1] = k % tiles_x;
+    items[3 * it + 2] = k / tiles_x;
+  }
+}
+}  // namespace
+
+static_assert(sizeof(ot_image) == sizeof(BImage) && offsetof(ot_image, pitch) == offsetof(BImage, pitch) &&
+                  offsetof(ot_image, offset) == offsetof(BImage, offset),
+              "the image table is copied to the device as it is");
+
+// Checks shared by ot_batch_create and ot_batch_run_static.
+static ot_status check_batch(const ot_params& p, int32_t n_images, const ot_image* images) {
+  if (p.stencil < 0 || p.stencil >= OT_NUM_STENCILS || kTraits[p.stencil].dims != 2)
+    return set_error(OT_EINVAL, "batches take a 2D stencil");
+  if (p.time_tile < 1 || p.time_tile > kMaxT || p.time_steps < 0 || p.block_x < 1 || p.block_y < 1 ||
+      p.elems_per_thread < 1)
+    return set_error(OT_EINVAL, "invalid time tile / steps / block");
+  if (p.phase_limit < 0 || p.phase_limit > 3) return set_error(OT_EINVAL, "phase limit must be 0..3");
+  if (p.variant & kProbeBits)
+    return set_error(OT_EUNSUPPORTED, "the timing probes run per grid (ot_plan_create), not in batches");
+  for (int i = 0; i < n_images; ++i) {
+    const ot_image& im = images[i];
+    if (im.width < 1 || im.height < 1 || im.pitch < im.width || im.offset < 0)
+      return set_error(OT_EINVAL, "image %d has an invalid layout", i);
   }
   return OT_OK;
 }
@@ -271,14 +385,8 @@ ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_im
   if (!params || !images || !out || n_images < 1) return set_error(OT_EINVAL, "bad arguments");
   *out = nullptr;
   const ot_params& p = *params;
-  if (p.stencil < 0 || p.stencil >= OT_NUM_STENCILS || kTraits[p.stencil].dims != 2)
-    return set_error(OT_EINVAL, "batches take a 2D stencil");
-  if (p.time_tile < 1 || p.time_tile > kMaxT || p.time_steps < 0 || p.block_x < 1 || p.block_y < 1 ||
-      p.elems_per_thread < 1)
-    return set_error(OT_EINVAL, "invalid time tile / steps / block");
-  if (p.phase_limit < 0 || p.phase_limit > 3) return set_error(OT_EINVAL, "phase limit must be 0..3");
-  if (p.variant & kProbeBits)
-    return set_error(OT_EUNSUPPORTED, "the timing probes run per grid (ot_plan_create), not in batches");
+  const ot_status cst = check_batch(p, n_images, images);
+  if (cst != OT_OK) return cst;
   const int32_t tile[3] = {p.block_x, p.block_y * p.elems_per_thread, 1};
   // items: every tile of every image, largest image first
   std::vector<int> order(n_images);
@@ -290,8 +398,6 @@ ot_status ot_batch_create(const ot_params* params, int32_t n_images, const ot_im
   std::vector<BImage> bim(n_images);
   for (int i = 0; i < n_images; ++i) {
     const ot_image& im = images[i];
-    if (im.width < 1 || im.height < 1 || im.pitch < im.width || im.offset < 0)
-      return set_error(OT_EINVAL, "image %d has an invalid layout", i);
     bim[i] = {im.width, im.height, im.pitch, 0, im.offset};
   }
   for (int i : order) {
@@ -426,6 +532,62 @@ ot_status ot_batch_run(ot_batch* b, float* const* rw_a, float* const* rw_b, cons
   OT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   tm.wall_s = std::chrono::duration<double>(t1 - t0).count();
   tm.event_s = ms * 1e-3;
+  if (timing) *timing = tm;
+  return OT_OK;
+}
+
+ot_status ot_batch_run_static(const ot_params* params, int32_t n_images, const ot_image* images,
+                              float* const* rw_a, float* const* rw_b, const float* const* ro, void* stream,
+                              ot_timing* timing) {
+  if (!params || !images || n_images < 1 || !rw_a || !rw_b) return set_error(OT_EINVAL, "bad arguments");
+  const ot_params& p = *params;
+  ot_status st = check_batch(p, n_images, images);
+  if (st != OT_OK) return st;
+  for (int k = 0; k < kTraits[p.stencil].num_rw; ++k)
+    if (!rw_a[k] || !rw_b[k]) return set_error(OT_EINVAL, "null rw arena %d", k);
+  for (int k = 0; k < kTraits[p.stencil].num_ro; ++k)
+    if (!ro || !ro[k]) return set_error(OT_EINVAL, "null read-only arena %d", k);
+  const int32_t tile[3] = {p.block_x, p.block_y * p.elems_per_thread, 1};
+  int64_t n_items = 0;
+  for (int i = 0; i < n_images; ++i) n_items += ceil_div(images[i].width, tile[0]) * ceil_div(images[i].height, tile[1]);
+  const int launches = (int)ceil_div(p.time_steps, p.time_tile);
+  // scratch layout: launch counters | image table | items (256-byte aligned parts)
+  const size_t counters_b = (size_t)round_up(std::max(1, launches) * 4, 256);
+  const size_t images_b = (size_t)round_up((int64_t)n_images * (int64_t)sizeof(BImage), 256);
+  const size_t need = counters_b + images_b + (size_t)n_items * 12;
+  if (n_images > kStaticMaxImages || n_items > INT32_MAX || need > kStaticScratchBytes)
+    return set_error(OT_ERESOURCE,
+                     "the batch's work queue (%d images, %lld tiles, %d launches: %zu bytes) exceeds the "
+                     "allocation-free entry point's fixed scratch (%d images, %zu bytes); use ot_batch_create",
+                     n_images, (long long)n_items, launches, need, kStaticMaxImages, kStaticScratchBytes);
+  ot_timing tm;
+  std::memset(&tm, 0, sizeof(tm));
+  tm.launches = launches;
+  tm.full_launches = p.time_steps / p.time_tile;
+  tm.final_buffer = launches % 2;
+  if (launches == 0) {
+    if (timing) *timing = tm;
+    return OT_OK;
+  }
+  LaunchShape full, last;
+  st = make_shape(p.stencil, p.time_tile, p.time_tile, p.elems_per_thread, tile, full, (int)n_items);
+  const int rem = p.time_steps % p.time_tile;
+  if (st == OT_OK) st = rem ? make_shape(p.stencil, p.time_tile, rem, p.elems_per_thread, tile, last, (int)n_items) : OT_OK;
+  if (st != OT_OK) return st;
+  if (rem == 0) last = full;
+  void* base = nullptr;
+  st = static_scratch(&base);
+  if (st != OT_OK) return st;
+  int32_t* counters = (int32_t*)base;
+  BImage* d_images = (BImage*)((unsigned char*)base + counters_b);
+  int32_t* items = (int32_t*)((unsigned char*)base + counters_b + images_b);
+  hipStream_t s = (hipStream_t)stream;
+  OT_HIP(hipMemcpyAsync(d_images, images, (size_t)n_images * sizeof(BImage), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(build_queue, dim3(1), dim3(1024), 0, s, (const BImage*)d_images, (int)n_images, tile[0], tile[1],
+                     items, (int)n_items, counters, launches);
+  OT_HIP(hipGetLastError());
+  st = enqueue_queue(p, launches, (int)n_items, full, last, d_images, items, counters, rw_a, rw_b, ro, s);
+  if (st != OT_OK) return st;
   if (timing) *timing = tm;
   return OT_OK;
 }

@@ -121,6 +121,8 @@ SIGNATURES = [
     ("ot_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("ot_batch_run", C.c_int, [C.c_void_p, _PP, _PP, _PP, C.c_void_p, C.c_int32, C.POINTER(Timing)]),
     ("ot_batch_destroy", None, [C.c_void_p]),
+    ("ot_batch_run_static", C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(Image), _PP, _PP, _PP, C.c_void_p,
+                                      C.POINTER(Timing)]),
     ("ot_fill_rician_phantom", C.c_int, [_FP, C.c_int32, C.POINTER(Image), C.c_uint64, C.c_float]),
     ("ot_dev_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     ("ot_dev_free", C.c_int, [C.c_void_p]),

@@ -296,6 +296,21 @@ ot_status ot_batch_info(const ot_batch* batch, int32_t* items, int32_t* workgrou
 ot_status ot_batch_run(ot_batch* batch, float* const* rw_a, float* const* rw_b,
                        const float* const* ro, void* stream, int32_t mode, ot_timing* timing);
 void ot_batch_destroy(ot_batch* batch);
+/* The same batched solve as ot_batch_create + ot_batch_run (work-queue
+ * launches, identical results) from an entry point that allocates nothing. No
+ * counterpart in the reference. For callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. The work queue (launch counters, image table, items) lives in a
+ * fixed 8 MiB device array that every GPU the library runs on holds until the
+ * process ends: at most 4096 images, and counters + table + 12 bytes per tile
+ * within 8 MiB (about 690,000 tiles); larger batches fail with OT_ERESOURCE.
+ * Enqueues on stream and returns without waiting (timing, optional, gets the
+ * launch counts and final_buffer); images is the host table of
+ * ot_batch_layout, copied at the call. Calls that overlap on one GPU share its
+ * array and race: order them on one stream or synchronise between them. */
+ot_status ot_batch_run_static(const ot_params* params, int32_t n_images, const ot_image* images,
+                              float* const* rw_a, float* const* rw_b, const float* const* ro, void* stream,
+                              ot_timing* timing);
 /* Seeded test images for the batch workload: per image a piecewise-constant
  * phantom of 8..32 random rectangles with intensities in [0,1], corrupted by
  * Rician noise |phantom + N(0,sigma) + i N(0,sigma)|. Writes the arena. */
