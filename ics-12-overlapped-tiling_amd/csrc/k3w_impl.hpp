@@ -140,6 +140,25 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   for (int l = 0; l < (kShareG ? T : 1); ++l)
 #pragma unroll
     for (int i = 0; i < (kShareG ? FR : 1); ++i) rg[l][0][i] = rg[l][1][i] = 0.0f;
+  // gradient-3d: each squared face difference (c - n)^2 equals (n - c)^2 of the
+  // neighbour, bit for bit (IEEE subtraction is antisymmetric), so a point
+  // computes only its +x, +y and +z differences and takes the -x one from the
+  // lane to its left (DPP), the -y one from the row above and the -z one from
+  // the previous plane, summed in the update's order: 3 subtractions and 3
+  // products per point instead of 6 and 6. dz keeps, per level, the +z
+  // difference of the plane stage s evaluates (slot PH % 2) and of the plane
+  // before it (the other slot); it is computed before the plane checks, since
+  // plane p+1 needs it even when p itself is not evaluated.
+  constexpr bool kShareD = ST == OT_G3D;
+  // 1e-4f held in a VGPR (the empty asm hides its value), so that the first
+  // add can take the left lane's difference as a DPP operand
+  float eps4 = 0.0001f;
+  if constexpr (kShareD) asm volatile("" : "+v"(eps4));
+  float dz[kShareD ? T : 1][2][kShareD ? FR : 1];
+#pragma unroll
+  for (int l = 0; l < (kShareD ? T : 1); ++l)
+#pragma unroll
+    for (int i = 0; i < (kShareD ? FR : 1); ++i) dz[l][0][i] = dz[l][1][i] = 0.0f;
 #pragma unroll
   for (int l = 0; l < T; ++l)
 #pragma unroll
@@ -254,6 +273,10 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
               rsq(((1e-9f + sq(q[s - 1][QN][i] - c)) + sq(q[s - 1][Q1][i + 1] - c)) + sq(from_right(c) - c));
         }
       }
+      if constexpr (kShareD) {
+#pragma unroll
+        for (int i = s; i <= FR - 1 - s; ++i) dz[s - 1][PH % 2][i] = sq(q[s - 1][Q1][i] - q[s - 1][QN][i]);
+      }
       if constexpr (FULL) {
         const int lo = z0 - (T - s), hi = z0 + TZ - 1 + (T - s);
         if (p < lo || p > hi) continue;
@@ -267,10 +290,21 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
           continue;
         }
       }
+      // gradient-3d: the +y difference of rows s-1 .. FR-1-s (row i-1's is row i's -y)
+      float dyb[kShareD ? FR : 1];
+      if constexpr (kShareD) {
+#pragma unroll
+        for (int i = s - 1; i <= FR - 1 - s; ++i) dyb[i] = sq(q[s - 1][Q1][i] - q[s - 1][Q1][i + 1]);
+      }
 #pragma unroll
       for (int i = s; i < FR - s; ++i) {
         float v;
-        if constexpr (L == 1) {
+        if constexpr (kShareD) {
+          const float c = q[s - 1][Q1][i];
+          const float dxr = sq(c - from_right(c));
+          v = c + rsq((((((from_left(dxr) + eps4) + dxr) + dyb[i - 1]) + dyb[i]) + dz[s - 1][(PH + 1) % 2][i]) +
+                      dz[s - 1][PH % 2][i]);
+        } else if constexpr (L == 1) {
           const float c = q[s - 1][Q1][i];
           const float zm = q[s - 1][Q2][i];
           const float zp = q[s - 1][QN][i];
