@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--no-t1", action="store_true", help="skip the T=1 comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotune / learned-pick leg")
     ap.add_argument("--images", type=int, default=256)
+    ap.add_argument("--no-per-image", action="store_true", help="batch workloads: skip the per-image baselines")
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only time a device copy of MIB MiB (HBM counter calibration)")
@@ -186,6 +187,14 @@ def _cpu_model():
     return platform.processor() or platform.machine()
 
 
+def batch_kernel_name(st, T, bx, by, e):
+    """The wave-per-tile batch kernel runs tiles whose x footprint fills a
+    wavefront with 16, 32 or 64 footprint rows; others run the LDS kernel."""
+    h = 2 * otlib.stencil_info(st).radius * T
+    wave = bx + h == 64 and e == 1 and by + h in (16, 32, 64)
+    return f"ot::k2w::k2w_batch<{otlib.STENCILS[st]}>" if wave else f"ot::k2d_batch<{otlib.STENCILS[st]}>"
+
+
 def bench_batch(args, stencil_name):
     """Batched Rician / TV denoising of uneven images: one persistent
     work-queue launch per time tile (hipGraph replay) against the same images
@@ -216,12 +225,24 @@ def bench_batch(args, stencil_name):
         otlib.check(otlib.lib().ot_dev_sync())
         return (time.perf_counter() - t0) / reps
 
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < args.settle:
+        bt.run([A], [B], RO, use_graph=True)
     batched = timed(bt, True, False, args.steps)
     batched_eager = timed(bt, False, False, args.steps)
-    per_image = timed(bt, False, True, max(2, args.steps // 2))
-    per_image_graph = timed(bt, True, True, max(2, args.steps // 2))
-    per_image_best = min(timed(bq, False, True, max(2, args.steps // 2)), timed(bq, True, True, max(2, args.steps // 2)))
+    # the kernels' own time: HIP events around each eager batched solve on its
+    # stream (ot_batch_run's EventElapsed), over the solve's launches
+    ev = [bt.run([A], [B], RO).event_s for _ in range(args.steps)]
+    launch_s = float(np.mean(ev)) / bt.launches
     pixels = float(sum(int(im.width) * int(im.height) for im in ims))
+    bytes_pp = 12 if st == otlib.RICIAN2D else 16  # u read + written, f (and V) read
+    bytes_per_launch = bytes_pp * pixels
+    achieved = bytes_per_launch / launch_s / 1e9
+    if not args.no_per_image:
+        per_image = timed(bt, False, True, max(2, args.steps // 2))
+        per_image_graph = timed(bt, True, True, max(2, args.steps // 2))
+        per_image_best = min(timed(bq, False, True, max(2, args.steps // 2)),
+                             timed(bq, True, True, max(2, args.steps // 2)))
     out = {
         "metric": f"batched {stencil_name} denoise Gpixel-iterations/s",
         "value": round(pixels * args.iterations / batched / 1e9, 3),
@@ -232,17 +253,27 @@ def bench_batch(args, stencil_name):
         "config": {"workload": f"{args.images} images {int(widths.min())}..{int(widths.max())} x "
                                f"{int(heights.min())}..{int(heights.max())}, {args.iterations} iterations",
                    "pixels": int(pixels), "time_tile": T, "tile": [bx, by * e], "elements_per_thread": e,
-                   "items": bt.items, "workgroups": bt.workgroups, "launches_per_step": bt.launches},
+                   "items": bt.items, "workgroups": bt.workgroups, "launches_per_step": bt.launches,
+                   "settle_s": args.settle},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": batch_kernel_name(st, T, bx, by, e),
+                     "avg_launch_us": round(launch_s * 1e6, 2),
+                     "timing": f"HIP events around each of {args.steps} eager batched solves, / {bt.launches} launches",
+                     "bytes_per_launch": bytes_per_launch},
         "batched_eager_ms": round(batched_eager * 1e3, 4),
-        "per_image_ms": round(per_image * 1e3, 4),
-        "per_image_graph_ms": round(per_image_graph * 1e3, 4),
-        "per_image_launches": int(len(ims) * bt.launches),
-        "speedup_vs_per_image": round(per_image / batched, 3),
-        "speedup_vs_per_image_graph": round(per_image_graph / batched, 3),
-        "per_image_best_config": {"time_tile": Tq, "tile": [bxq, byq * eq], "elements_per_thread": eq,
-                                  "ms": round(per_image_best * 1e3, 4)},
-        "speedup_vs_per_image_best": round(per_image_best / batched, 3),
     }
+    if not args.no_per_image:
+        out.update({
+            "per_image_ms": round(per_image * 1e3, 4),
+            "per_image_graph_ms": round(per_image_graph * 1e3, 4),
+            "per_image_launches": int(len(ims) * bt.launches),
+            "speedup_vs_per_image": round(per_image / batched, 3),
+            "speedup_vs_per_image_graph": round(per_image_graph / batched, 3),
+            "per_image_best_config": {"time_tile": Tq, "tile": [bxq, byq * eq], "elements_per_thread": eq,
+                                      "ms": round(per_image_best * 1e3, 4)},
+            "speedup_vs_per_image_best": round(per_image_best / batched, 3),
+        })
     bt.close()
     bq.close()
     print(json.dumps(out), flush=True)
