@@ -126,6 +126,10 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   uint32_t vld[kSoff ? FR : 1];
 #pragma unroll
   for (int i = 0; i < (kSoff ? FR : 1); ++i) vld[i] = lane_ld + row_ld[i];
+  // lane + row term of every output row, for the steady-state stores
+  uint32_t vst[kSoff ? FR : 1];
+#pragma unroll
+  for (int i = 0; i < (kSoff ? FR : 1); ++i) vst[i] = (i >= T && i < FR - T) ? lane_st + row_st[i] : 0u;
 
   float q[T][NQ][FR];
   float fq[T + 1][FR];
@@ -232,6 +236,24 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
                                             (int)(lane_st + (row_st[i] + pterm)), 0, 0);
   };
 
+  // flush() in a steady-state iteration k: the pending plane is k - 1 - T,
+  // whose offset goes in the scalar offset; the vector offset is the lane +
+  // row term computed once per wave, so no vector instruction per store. A
+  // lane or row outside the domain has vector offset >= range and its store
+  // is dropped with or without the scalar offset. rd_steady is the output
+  // descriptor, except in the first steady iteration when the checked
+  // iteration before it produced no output plane: it is then a descriptor of
+  // range 0, through which every store is dropped.
+  const __amdgpu_buffer_rsrc_t rnull = __builtin_amdgcn_make_buffer_rsrc((void*)f.dst[0], (short)0, 0, 0x00020000);
+  __amdgpu_buffer_rsrc_t rd_steady = rdst;
+  auto flush_steady = [&](int k) __attribute__((always_inline)) {
+    const int soff = (int)((uint32_t)(k - 1 - z0) * plane_bytes);
+#pragma unroll
+    for (int i = T; i < FR - T; ++i)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, outv[i]), rd_steady, (int)vst[i], soff, 0);
+    rd_steady = rdst;
+  };
+
   auto iter = [&](auto phc, auto fullc, const int k) __attribute__((always_inline)) {
     constexpr int PH = decltype(phc)::value;
     constexpr bool FULL = decltype(fullc)::value != 0;
@@ -252,7 +274,8 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
       q[0][QN][i] = pf[PS][i];
       if constexpr (RHS) fq[0][i] = pfr[PS][i];
     }
-    flush();
+    if constexpr (FULL || !kSoff) flush();
+    else flush_steady(k);
     out_p = -1;
     fetch(k + PD, IC<PS>{}, fullc);
 
@@ -379,6 +402,10 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   // checked blocks until one fits the steady range, then steady blocks; the
   // steady range ends at kend, so only the remainder below is checked again
   for (; k + U - 1 <= kend && !(k >= k_lo && k + U - 1 <= k_fast_end); k += U) block(IC<1>{}, k);
+  if constexpr (kSoff) {
+    // a checked iteration k - 1 >= z0 + T - 1 leaves plane k - 1 - T pending, or none
+    rd_steady = out_p >= 0 ? rdst : rnull;
+  }
   if constexpr (kSteady)
     for (; k + U - 1 <= k_fast_end; k += U) block(IC<0>{}, k);
   if (k <= kend) iter(IC<0>{}, IC<1>{}, k);
