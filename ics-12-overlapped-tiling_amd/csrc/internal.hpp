@@ -142,6 +142,9 @@ ot_status launch_3dw_p3d27(int t_eff, int rows, const LaunchGeom& g, const Field
 // Raise the dynamic-LDS limit of a kernel that needs more than 64 KiB.
 ot_status prepare_lds(const void* kernel, int lds_bytes);
 
+// SIMDs of the current device (4 per compute unit).
+int simd_count();
+
 // Launch shapes (host-side, also used by ot_derive).
 void shape_1d(int stencil, int T, int ept, const int32_t tile[3], int* threads, int* lds);
 void shape_2d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* by, int* lds);

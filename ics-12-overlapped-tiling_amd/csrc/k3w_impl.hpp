@@ -20,6 +20,11 @@
 //   * Chunks away from the z faces run their middle iterations as unrolled
 //     blocks without the per-stage range and domain checks (the checks are
 //     scalar work that each wave issues in order with its vector work).
+//   * When the whole grid is resident at once with at most three waves per
+//     SIMD, each wave lowers its issue priority as it completes blocks, so the
+//     waves of a SIMD advance together instead of oldest first (which leaves
+//     the youngest wave alone, unable to hide memory latency, for the last
+//     third of the launch).
 // Out-of-domain points are forced to +0 at every level (Dirichlet 0).
 #pragma once
 #include "internal.hpp"
@@ -50,7 +55,7 @@ struct KW {
 constexpr int kWaves = 4;
 
 template <int ST, int T, int FR>
-__global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
+__global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, int balance) {
   constexpr int L = KW<ST>::L;
   constexpr bool RHS = KW<ST>::RHS;
   constexpr int NQ = KW<ST>::NQ;
@@ -388,7 +393,16 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f) {
   const bool interior = kSteady && g.phase_limit == 0 && z0 - T >= 0 && z0 + TZ - 1 + T < g.n[2];
   const int k_lo = z0 + T;
   const int k_fast_end = interior ? kend : kstart - 1;
+  // issue priority 3, 2, 1, 0 for the first, second, third and later blocks
+  int blocks_done = 0;
   auto block = [&](auto fullc, const int k) __attribute__((always_inline)) {
+    if (balance) {
+      const int b = blocks_done++;
+      if (b == 0) __builtin_amdgcn_s_setprio(3);
+      else if (b == 1) __builtin_amdgcn_s_setprio(2);
+      else if (b == 2) __builtin_amdgcn_s_setprio(1);
+      else __builtin_amdgcn_s_setprio(0);
+    }
     iter(IC<0>{}, fullc, k);
     iter(IC<1>{}, fullc, k + 1);
     iter(IC<2>{}, fullc, k + 2);
@@ -427,7 +441,11 @@ ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool p
   if (prepare) return prepare_lds((const void*)&k3w<ST, T, FR>, 0);
   const int nw = g.grid[0] * g.grid[1] * g.grid[2];
   const int nblk = (nw + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL((k3w<ST, T, FR>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f);
+  // progress balancing pays only when every wave is resident from the start
+  // and a SIMD holds at most three of them (measured: with a second round of
+  // waves, or five memory-bound waves per SIMD, oldest-first issue is faster)
+  const int balance = nw <= 3 * simd_count() ? 1 : 0;
+  hipLaunchKernelGGL((k3w<ST, T, FR>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f, balance);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return set_error(OT_EHIP, "3D wave kernel launch failed: %s", hipGetErrorString(err));
   return OT_OK;

@@ -8,6 +8,7 @@
 //   throwOnError                                            -> ot_status + ot_last_error
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -46,6 +47,20 @@ ot_status prepare_lds(const void* kernel, int lds_bytes) {
   hipFuncAttributes attr;
   if (hipFuncGetAttributes(&attr, kernel) == hipSuccess) g_prepared_regs = attr.numRegs;
   return OT_OK;
+}
+
+int simd_count() {
+  static std::atomic<int> cached[64];  // per device ordinal, 0 = not queried yet
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+  if (dev < 64) {
+    const int c = cached[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 0;  // unknown: callers then take no decision that depends on it
+  if (dev < 64) cached[dev].store(4 * cus, std::memory_order_relaxed);
+  return 4 * cus;
 }
 
 void shape_3d(int stencil, int T, int ept, const int32_t tile[3], int* bx, int* by, int* lds) {
