@@ -5,11 +5,11 @@ Workload (default): Jacobi-3D 7-point FP32 on a 256^3 grid, 32 time steps,
 interior values uniform [0,1) from a seed, Dirichlet-0 boundary, time-tiled with
 the autotuned tile configuration of workloads.py. One bench "step" is one
 whole 32-time-step solve of the grid (ceil(32/T) kernel launches), with the
-fields resident in HBM. After 0.25 s of untimed solves (the GPU's clocks
-settle; --settle) and W warm-up solves, the K timed solves are enqueued back
-to back between two device synchronisations, once as hipGraph replays and once
-as eager launches, and the faster of the two is the headline (both are
-reported).
+fields resident in HBM. The K timed solves are enqueued back to back between
+two device synchronisations, once as hipGraph replays and once as eager
+launches, each after 0.25 s of untimed solves in that mode (the GPU's clocks
+settle; --settle) and W warm-up solves, and the faster of the two is the
+headline (both are reported).
 
 Prints one JSON line with the metric (GStencil/s = grid points x time steps /
 second), the roofline of the dominant kernel (algorithmic HBM bytes per launch
@@ -48,7 +48,7 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed solves")
-    ap.add_argument("--warmup", type=int, default=3, help="untimed solves")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed solves")
     ap.add_argument("--settle", type=float, default=0.25,
                     help="seconds of untimed back-to-back solves before the warm-up solves (clock settling)")
     ap.add_argument("--workload", default="jacobi3d", choices=list(WORKLOADS) + list(DEFAULT_BATCH))
@@ -97,17 +97,20 @@ def solve_rate(w, cfg, n3, steps, K, W, settle_s):
     B = [otlib.DeviceBuffer.from_array(x) for x in rw]
     R = [otlib.DeviceBuffer.from_array(x) for x in ro]
     out = {"launches": d.launches, "derived": d}
-    # settle: the GPU reaches its steady clocks after some tens of milliseconds
-    # of continuous work (launches early in a run are up to 10% slower), so
-    # solves run back to back for settle_s before the counted warm-up solves
-    t_settle = time.perf_counter()
-    while time.perf_counter() - t_settle < settle_s:
-        for _ in range(8):
-            plan.run(A, B, R, enqueue_only=True)
-        otlib.check(otlib.lib().ot_dev_sync())
     for mode, graph, stream in (("graph", True, plan.stream()), ("eager", False, None)):
+        # settle: the GPU reaches its steady clocks after some tens of
+        # milliseconds of continuous work (launches early in a run are up to
+        # 10% slower), and the first solves after a switch between graph
+        # replays and eager launches run slower too, so solves of this mode run
+        # back to back for settle_s before the counted warm-up solves
+        t_settle = time.perf_counter()
+        while time.perf_counter() - t_settle < settle_s:
+            for _ in range(8):
+                plan.run(A, B, R, use_graph=graph, enqueue_only=True)
+            otlib.check(otlib.lib().ot_dev_sync())
+        # warm-ups back to back, like the timed solves
         for _ in range(W):
-            plan.run(A, B, R, use_graph=graph)
+            plan.run(A, B, R, use_graph=graph, enqueue_only=True)
         timer = otlib.StreamTimer(stream)
         otlib.check(otlib.lib().ot_dev_sync())
         t0 = time.perf_counter()
