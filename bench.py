@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-t1", action="store_true", help="skip the T=1 comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotune / learned-pick leg")
+    ap.add_argument("--no-balance", action="store_true",
+                    help="3D wave kernels: oldest-first issue, no progress balancing (OT_VARIANT_NO_BALANCE)")
     ap.add_argument("--images", type=int, default=256)
     ap.add_argument("--no-per-image", action="store_true", help="batch workloads: skip the per-image baselines")
     ap.add_argument("--iterations", type=int, default=50)
@@ -81,7 +83,7 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
 
 
-def solve_rate(w, cfg, n3, steps, K, W, settle_s):
+def solve_rate(w, cfg, n3, steps, K, W, settle_s, variant=0):
     """Time K whole solves after W warm-ups, enqueued back to back between two
     device synchronisations, once as hipGraph replays (on the plan's stream)
     and once as eager launches (on the null stream). Each timed region is also
@@ -90,6 +92,7 @@ def solve_rate(w, cfg, n3, steps, K, W, settle_s):
     the eager kernel-only seconds per launch (one event pair per launch);
     launches per solve and the derived geometry."""
     p = plan_params(w, cfg, n3, steps)
+    p.variant = variant
     plan = otlib.Plan(p)
     d = plan.derived
     rw, ro = make_inputs(w, d)
@@ -370,7 +373,8 @@ def main():
     if args.T:
         tiled = dict(T=args.T, x=args.bx, y=args.by, z=args.bz or 1, e=args.ept or 1)
 
-    r = solve_rate(w, tiled, n3, steps_t, args.steps, args.warmup, args.settle)
+    variant = otlib.VARIANT_NO_BALANCE if args.no_balance else 0
+    r = solve_rate(w, tiled, n3, steps_t, args.steps, args.warmup, args.settle, variant)
     d, launches = r["derived"], r["launches"]
     # back-to-back solves from graph replays or from eager launches: the faster
     # of the two (host clock around the K solves) is the headline, both are
@@ -408,7 +412,7 @@ def main():
                    "elements_per_thread": tiled["e"], "launches_per_step": launches, "launch_mode": mode,
                    "ms_per_step_graph": round(r["graph"]["wall_s"] * 1e3, 4),
                    "ms_per_step_eager": round(r["eager"]["wall_s"] * 1e3, 4),
-                   "settle_s": args.settle},
+                   "settle_s": args.settle, "issue_balance": not args.no_balance},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel_name(w, d), "avg_launch_us": round(launch_s * 1e6, 2),
@@ -422,7 +426,7 @@ def main():
     }
     if not args.no_t1:
         t1 = w["t1"]
-        r1 = solve_rate(w, t1, n3, steps_t, args.steps, args.warmup, args.settle)
+        r1 = solve_rate(w, t1, n3, steps_t, args.steps, args.warmup, args.settle, variant)
         m1 = "graph" if r1["graph"]["wall_s"] <= r1["eager"]["wall_s"] else "eager"
         sec1 = r1[m1]["wall_s"]
         g1 = points * steps_t / sec1 / 1e9

@@ -54,8 +54,16 @@ struct KW {
 // Waves per workgroup (the workgroup only bundles independent waves).
 constexpr int kWaves = 4;
 
-template <int ST, int T, int FR>
+// CLK: per-wave clock records (OT_VARIANT_DUMP_CLOCKS), a separate
+// instantiation so the production kernels carry none of its code.
+template <int ST, int T, int FR, bool CLK = false>
 __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, int balance) {
+  uint64_t clk[5] = {0, 0, 0, 0, 0};
+  uint64_t rt0 = 0;
+  if constexpr (CLK) {
+    clk[0] = __builtin_amdgcn_s_memtime();
+    rt0 = __builtin_amdgcn_s_memrealtime();
+  }
   constexpr int L = KW<ST>::L;
   constexpr bool RHS = KW<ST>::RHS;
   constexpr int NQ = KW<ST>::NQ;
@@ -416,12 +424,14 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, in
   // checked blocks until one fits the steady range, then steady blocks; the
   // steady range ends at kend, so only the remainder below is checked again
   for (; k + U - 1 <= kend && !(k >= k_lo && k + U - 1 <= k_fast_end); k += U) block(IC<1>{}, k);
+  if constexpr (CLK) clk[1] = __builtin_amdgcn_s_memtime();
   if constexpr (kSoff) {
     // a checked iteration k - 1 >= z0 + T - 1 leaves plane k - 1 - T pending, or none
     rd_steady = out_p >= 0 ? rdst : rnull;
   }
   if constexpr (kSteady)
     for (; k + U - 1 <= k_fast_end; k += U) block(IC<0>{}, k);
+  if constexpr (CLK) clk[2] = __builtin_amdgcn_s_memtime();
   if (k <= kend) iter(IC<0>{}, IC<1>{}, k);
   if (k + 1 <= kend) iter(IC<1>{}, IC<1>{}, k + 1);
   if (k + 2 <= kend) iter(IC<2>{}, IC<1>{}, k + 2);
@@ -430,6 +440,26 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, in
     if (k + 4 <= kend) iter(IC<4>{}, IC<1>{}, k + 4);
   }
   flush();
+  if constexpr (CLK) {
+    clk[3] = __builtin_amdgcn_s_memtime();
+    uint32_t hw, xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    clk[4] = __builtin_amdgcn_s_memtime();
+    const uint64_t rt1 = __builtin_amdgcn_s_memrealtime();
+    if (lane == 0) {  // one vector store per field of the wave's record
+      ot_clock_record& r = g.clocks[wid];
+      r.block_x = bxi;
+      r.block_y = byi;
+      r.wave = bzi;
+      r.hw_id = hw;
+      r.xcc_id = xcc;
+      r.valid = 1;
+      for (int j = 0; j < 5; ++j) r.clock[j] = clk[j];
+      r.realtime[0] = rt0;
+      r.realtime[1] = rt1;
+    }
+  }
 }
 
 template <int ST, int T, int FR>
@@ -438,13 +468,30 @@ ot_status launch_t(const LaunchGeom& g, const FieldPtrs& f, void* stream, bool p
   if ((int64_t)(g.tile[2] + 2 * T + 3) * plane_bytes >= (1LL << 30))
     return set_error(OT_EUNSUPPORTED, "3D plane of %lld bytes too large for a z-chunk of %d planes",
                      (long long)plane_bytes, g.tile[2]);
-  if (prepare) return prepare_lds((const void*)&k3w<ST, T, FR>, 0);
+  // clock records exist for jacobi-3d (the plan refuses them elsewhere)
+  constexpr bool kClk = ST == OT_J3D;
+  if (g.clocks && !kClk) return set_error(OT_EUNSUPPORTED, "3D clock records exist for jacobi-3d only");
+  if (prepare) {
+    if constexpr (kClk)
+      if (g.clocks) return prepare_lds((const void*)&k3w<ST, T, FR, kClk>, 0);
+    return prepare_lds((const void*)&k3w<ST, T, FR>, 0);
+  }
   const int nw = g.grid[0] * g.grid[1] * g.grid[2];
   const int nblk = (nw + kWaves - 1) / kWaves;
   // progress balancing pays only when every wave is resident from the start
   // and a SIMD holds at most three of them (measured: with a second round of
   // waves, or five memory-bound waves per SIMD, oldest-first issue is faster)
-  const int balance = nw <= 3 * simd_count() ? 1 : 0;
+  const int balance = (!(g.variant & OT_VARIANT_NO_BALANCE) && nw <= 3 * simd_count()) ? 1 : 0;
+  if constexpr (kClk) {
+    if (g.clocks) {
+      hipLaunchKernelGGL((k3w<ST, T, FR, kClk>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f,
+                         balance);
+      const hipError_t err = hipGetLastError();
+      if (err != hipSuccess)
+        return set_error(OT_EHIP, "3D wave kernel launch failed: %s", hipGetErrorString(err));
+      return OT_OK;
+    }
+  }
   hipLaunchKernelGGL((k3w<ST, T, FR>), dim3(nblk), dim3(kWaves * 64), 0, (hipStream_t)stream, g, f, balance);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return set_error(OT_EHIP, "3D wave kernel launch failed: %s", hipGetErrorString(err));

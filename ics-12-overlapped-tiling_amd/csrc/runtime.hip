@@ -290,10 +290,15 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
                                        ? -1
                                        : k2w_rows(params->stencil, params->time_tile, d.tile))
                                 : 0;
-  if ((params->variant & kProbeBits) && d.dims != 2) {
+  // timing probes: 2D LDS kernel (checked at launch); clock records also for
+  // the wave-pipelined jacobi-3d kernel
+  const bool clocks_3dw = d.dims == 3 && params->stencil == OT_J3D && g.wave_rows > 0 &&
+                          (params->variant & kProbeBits) == OT_VARIANT_DUMP_CLOCKS;
+  if ((params->variant & kProbeBits) && d.dims != 2 && !clocks_3dw) {
     delete pl;
     return set_error(OT_EUNSUPPORTED,
-                     "--time-only and --dump-clocks exist for jacobi-2d and gradient-2d (as in the reference)");
+                     "--time-only and --dump-clocks exist for jacobi-2d and gradient-2d (as in the reference); "
+                     "--dump-clocks also for jacobi-3d tiles of the wave-pipelined kernel");
   }
   // Check the instantiations this plan will launch (full T and the shortened
   // last launch) exist, and raise their LDS limits now.
@@ -307,7 +312,9 @@ ot_status ot_plan_create(const ot_params* params, ot_plan** plan) {
   if (st == OT_OK && last > 0)
     st = launch_any(params->stencil, d.dims, last, params->elems_per_thread, g, f, nullptr, true);
   if (st == OT_OK && (params->variant & OT_VARIANT_DUMP_CLOCKS)) {
-    pl->nclocks = (int64_t)d.grid_blocks[0] * d.grid_blocks[1] * kClockSlotsPerBlock;
+    // 2D: a fixed slot per (workgroup, wave); 3D wave kernel: one per wave
+    pl->nclocks = clocks_3dw ? (int64_t)d.grid_blocks[0] * d.grid_blocks[1] * d.grid_blocks[2]
+                             : (int64_t)d.grid_blocks[0] * d.grid_blocks[1] * kClockSlotsPerBlock;
     const size_t bytes = (size_t)pl->nclocks * sizeof(ot_clock_record);
     hipError_t e = hipMalloc((void**)&pl->clocks, bytes);
     if (e == hipSuccess) e = hipMemset(pl->clocks, 0, bytes);

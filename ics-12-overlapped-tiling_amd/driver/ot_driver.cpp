@@ -18,9 +18,11 @@
 //     error;
 //   * --time-only gmem|instr|smem and -c/--dump-clocks run timing-probe
 //     instantiations of the 2D LDS kernel (jacobi-2d and gradient-2d, the
-//     drivers that have --time-only); the clock CSV goes to stderr as in the
-//     reference (jacobi-2d.cpp:957-997), one row per wave of one compute unit,
-//     stamps from s_memtime; --break-threads is accepted and has no effect;
+//     drivers that have --time-only); -c also of the wave-pipelined jacobi-3d
+//     kernel (bx, by = the wave's x and y tile, warp = its z-chunk); the clock
+//     CSV goes to stderr as in the reference (jacobi-2d.cpp:957-997), one row
+//     per wave of one compute unit, stamps from s_memtime; --break-threads is
+//     accepted and has no effect;
 //   * a malformed option value is an error (usage, exit status 1).
 #include <algorithm>
 #include <chrono>
@@ -98,7 +100,8 @@ int usage(const char* prog) {
                "  -f [ --load-kernel ] arg           Load plan description from disk\n"
                "  -w [ --save-kernel ] arg           Save plan description to disk\n"
                "  -v [ --verify ]                    Verify results\n"
-               "  -c [ --dump-clocks ] arg (=0)      Dump per-wave phase clocks to stderr (2D LDS kernel)\n"
+               "  -c [ --dump-clocks ] arg (=0)      Dump per-wave phase clocks to stderr (2D LDS kernel,\n"
+               "                                     jacobi-3d wave kernel)\n"
                "  -b [ --break-threads ]             Accepted for compatibility\n"
                "  --time-only arg                    Time one phase only: gmem | instr | smem (2D LDS kernel)\n"
                "  --stencil arg                      Stencil name (default: from program name)\n"

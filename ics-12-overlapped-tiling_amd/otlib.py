@@ -25,6 +25,7 @@ STENCILS = [
 GEOM_TILE, GEOM_REFERENCE = 0, 1
 VARIANT_FDTD_PHYSICAL = 1
 VARIANT_TIME_ONLY_GMEM, VARIANT_TIME_ONLY_INSTR, VARIANT_TIME_ONLY_SMEM, VARIANT_DUMP_CLOCKS = 2, 4, 8, 16
+VARIANT_NO_BALANCE = 32
 
 OT_OK, OT_EINVAL, OT_ERESOURCE, OT_EHIP, OT_EUNSUPPORTED = range(5)
 

@@ -90,8 +90,13 @@ enum {
   OT_VARIANT_TIME_ONLY_SMEM = 8,
   /* Per-wave clock stamps at the kernel's phase boundaries (-c/--dump-clocks,
    * jacobi-2d.cpp:202-205, :453-509, :957-997), read with ot_plan_clocks. 2D
-   * LDS kernel of jacobi-2d and gradient-2d; results are unchanged. */
-  OT_VARIANT_DUMP_CLOCKS = 16
+   * LDS kernel of jacobi-2d and gradient-2d, and the wave-pipelined 3D kernel
+   * of jacobi-3d (an extension: the reference stamps its 2D kernels only);
+   * results are unchanged. */
+  OT_VARIANT_DUMP_CLOCKS = 16,
+  /* Wave-pipelined 3D kernels: no issue-priority progress balancing (the
+   * hardware's oldest-first issue order); results are unchanged. */
+  OT_VARIANT_NO_BALANCE = 32
 };
 
 typedef struct ot_stencil_info {
@@ -240,7 +245,11 @@ ot_status ot_launch(ot_plan* plan, const float* const* rw_src, float* const* rw_
  * s_memtime (shader clock) at P0 kernel entry, P1 after the index set-up, P2
  * after the global loads and the LDS publish, P3 after the on-chip time steps
  * and P4 after the stores, plus s_memrealtime (constant 100 MHz) at P0 and P4
- * and the hardware ids of the wave's slot. */
+ * and the hardware ids of the wave's slot.
+ * Wave-pipelined 3D kernel (one record per wave, slot = wave index): block_x,
+ * block_y = the wave's x and y tile, wave = its z-chunk; P1 after the
+ * pipeline fill (the first unrolled block of planes), P2 after the steady-state
+ * blocks, P3 after the last plane's stores are issued, P4 at exit. */
 typedef struct ot_clock_record {
   int32_t block_x, block_y; /* output tile of the workgroup */
   int32_t wave;             /* wavefront index inside the workgroup */
@@ -252,9 +261,10 @@ typedef struct ot_clock_record {
 } ot_clock_record;
 
 /* Copy the clock records of the plan's last launch to host memory: count
- * receives the number of record slots (16 per workgroup, slot 16 * workgroup +
- * wave), out gets min(capacity, count) of them (out may be NULL with capacity
- * 0). Slots whose valid is 0 had no wavefront in the last launch. */
+ * receives the number of record slots (2D: 16 per workgroup, slot 16 *
+ * workgroup + wave; 3D: one per wave), out gets min(capacity, count) of them
+ * (out may be NULL with capacity 0). Slots whose valid is 0 had no wavefront
+ * in the last launch. */
 ot_status ot_plan_clocks(ot_plan* plan, ot_clock_record* out, int32_t capacity, int32_t* count);
 
 /* Run the same problem with straightforward one-step-per-launch kernels (no

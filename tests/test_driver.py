@@ -43,11 +43,16 @@ def test_driver_malformed_value_prints_usage_and_exits_1(args):
 
 
 def test_time_only_and_dump_clocks_exist_only_for_the_2d_lds_drivers():
-    # refused before any device work: the 3D drivers have no timing probes
+    # refused before any device work: the 3D drivers have no --time-only, and
+    # clock records only for jacobi-3d tiles of the wave-pipelined kernel (the
+    # reference geometry's default tile is not one)
     rc, keys, r = run_driver("jacobi-3d", "-n", 32, "--time-only", "gmem")
     assert rc == 1 and "jacobi-2d and gradient-2d" in r.stdout
     rc, keys, r = run_driver("jacobi-3d", "-n", 32, "-c", "1")
     assert rc == 1 and "jacobi-2d and gradient-2d" in r.stdout
+    rc, keys, r = run_driver("poisson-3d-rhs", "-n", 64, "-s", 2, "-x", 60, "-y", 4, "-z", 8, "--geometry", "tile",
+                             "-c", "1")
+    assert rc == 1 and "jacobi-3d tiles of the wave-pipelined kernel" in r.stdout
 
 
 def test_every_stencil_has_a_driver(otlib):

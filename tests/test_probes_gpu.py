@@ -1,11 +1,13 @@
-"""Timing probes of the 2D LDS kernel through the C ABI (MI355X): the
-reference's --time-only modes (jacobi-2d.cpp:258-266, :296-341, :368-433,
-:462-481) and its per-warp phase clocks (--dump-clocks, :202-205, :453-509).
+"""Timing probes through the C ABI (MI355X): the reference's --time-only modes
+of the 2D LDS kernel (jacobi-2d.cpp:258-266, :296-341, :368-433, :462-481) and
+its per-warp phase clocks (--dump-clocks, :202-205, :453-509), the latter also
+for the wave-pipelined jacobi-3d kernel.
 
   * clock stamps change no result bit, one record per wavefront of the last
     launch, every stamp sequence in phase order;
   * gmem copies the loaded footprint's tile interior back out (so a whole
-    solve leaves the input unchanged), instr and smem store nothing.
+    solve leaves the input unchanged), instr and smem store nothing;
+  * the 3D wave kernel's issue-priority balancing changes no result bit.
 """
 import ctypes as C
 
@@ -110,3 +112,76 @@ def test_probes_are_refused_where_the_reference_has_none(otlib):
         with pytest.raises(otlib.OtError) as e:
             otlib.Plan(_p(st, n3, 4, 2, blk, 1, variant=otlib.VARIANT_TIME_ONLY_GMEM))
         assert e.value.status == otlib.OT_EUNSUPPORTED
+    # 3D: clock records for jacobi-3d wave tiles only, no --time-only
+    for st, blk, bit in [(otlib.P3D7, (60, 4, 8), otlib.VARIANT_DUMP_CLOCKS),
+                         (otlib.J3D, (16, 8, 8), otlib.VARIANT_DUMP_CLOCKS),
+                         (otlib.J3D, (60, 4, 8), otlib.VARIANT_TIME_ONLY_INSTR)]:
+        p = otlib.make_params(st, nx=64, ny=64, nz=64, problem_size=64, steps=4, T=2, x=blk[0], y=blk[1], z=blk[2],
+                              e=1, variant=bit)
+        with pytest.raises(otlib.OtError) as e:
+            otlib.Plan(p)
+        assert e.value.status == otlib.OT_EUNSUPPORTED
+
+
+def _j3d(n3, steps, T, blk, variant=0):
+    return otlib.make_params(otlib.J3D, nx=n3[0], ny=n3[1], nz=n3[2], problem_size=n3[0], steps=steps, T=T,
+                             x=blk[0], y=blk[1], z=blk[2], e=1, variant=variant)
+
+
+def _solve(p, rw):
+    plan = otlib.Plan(p)
+    d = plan.derived
+    full = otlib.empty_field(d)
+    full[otlib.interior_slices(d)] = rw[0]
+    A, B = otlib.DeviceBuffer.from_array(full), otlib.DeviceBuffer.from_array(full)
+    tm = plan.run([A], [B])
+    out = (B if tm.final_buffer else A).download(full.shape)[otlib.interior_slices(d)]
+    A.free()
+    B.free()
+    return plan, out
+
+
+# wave tiles (x + 2T = 64): the last launch of 7 steps at T=3 is a T=1 launch
+CASES_3D = [((100, 70, 45), 7, 3, (58, 6, 10)), ((130, 64, 64), 6, 2, (60, 4, 19)), ((64, 50, 33), 5, 4, (56, 8, 9))]
+
+
+@pytest.mark.parametrize("case", CASES_3D, ids=[f"T{c[2]}-{c[0][0]}x{c[0][1]}x{c[0][2]}" for c in CASES_3D])
+def test_3d_wave_clock_records(otlib, case):
+    n3, steps, T, blk = case
+    p0 = _j3d(n3, steps, T, blk)
+    d = otlib.derive(p0)
+    rw, _ = random_inputs(otlib.J3D, domain_shape(d), seed=6)
+    print(f"3D clocks {n3} T={T} {blk} steps {steps}", flush=True)
+    _, ref = _solve(p0, rw)
+    plan, out = _solve(_j3d(n3, steps, T, blk, variant=otlib.VARIANT_DUMP_CLOCKS), rw)
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    g = plan.derived.grid_blocks
+    n = C.c_int32()
+    otlib.check(otlib.lib().ot_plan_clocks(plan.handle, None, 0, C.byref(n)))
+    assert n.value == g[0] * g[1] * g[2]
+    recs = (ClockRecord * n.value)()
+    otlib.check(otlib.lib().ot_plan_clocks(plan.handle, recs, n.value, C.byref(n)))
+    seen = set()
+    for i, r in enumerate(recs):
+        assert r.valid == 1
+        # record i is wave i: x tile fastest, then y tile, then z-chunk
+        assert (r.block_x, r.block_y, r.wave) == (i % g[0], (i // g[0]) % g[1], i // (g[0] * g[1]))
+        c = list(r.clock)
+        assert c[0] <= c[1] <= c[2] <= c[3] <= c[4], c
+        assert r.realtime[1] >= r.realtime[0] > 0
+        seen.add((r.block_x, r.block_y, r.wave))
+    assert len(seen) == n.value
+    plan.close()
+
+
+def test_3d_wave_balancing_changes_no_result_bit(otlib):
+    n3, steps, T, blk = (256, 96, 80), 8, 3, (58, 6, 19)
+    p = _j3d(n3, steps, T, blk)
+    d = otlib.derive(p)
+    rw, _ = random_inputs(otlib.J3D, domain_shape(d), seed=8)
+    print(f"3D balance {n3} T={T} {blk}", flush=True)
+    a = _solve(p, rw)
+    b = _solve(_j3d(n3, steps, T, blk, variant=otlib.VARIANT_NO_BALANCE), rw)
+    a[0].close()
+    b[0].close()
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
