@@ -83,6 +83,11 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
     mxy[i] = (in_x && row_in) ? 0xFFFFFFFFu : 0u;
   }
   if (g.phase_limit == 1) return;
+  // gradient-2d: 1e-4f held in a VGPR (the empty asm hides its value), so the
+  // first add of the update can take the left lane's difference as a DPP
+  // operand; set once, ahead of the loads, so the asm orders nothing else
+  float eps4 = 0.0001f;
+  if constexpr (ST == OT_G2D) asm volatile("" : "+v"(eps4));
 
   // A tile whose whole footprint lies in the domain needs no Dirichlet mask;
   // each variant holds its own registers from load to store.
@@ -140,6 +145,14 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
             gr[i] = rsq((1e-8f + sq(u[0][i + 1] - c)) + sq(from_right(c) - c));
           }
         }
+        // gradient-2d: each squared face difference (c - n)^2 equals (n - c)^2
+        // of the neighbour bit for bit (IEEE subtraction is antisymmetric), so
+        // a point computes its +x and +y differences and takes the -x one from
+        // the lane to its left (DPP) and the -y one from the row above (dyt,
+        // carried down the rows), summed in the update's order
+        // (gradient-2d.cpp:691-699)
+        float dyt = 0.0f;
+        if constexpr (ST == OT_G2D) dyt = sq(u[0][s - 1] - u[0][s]);
 #pragma unroll
         for (int i = s; i < FR - s; ++i) {
           const float c = u[0][i], tc = u[0][i - 1], bc = u[0][i + 1];
@@ -147,7 +160,10 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
           if constexpr (ST == OT_J2D) {
             v = j2d(from_left(c), c, from_right(c), tc, bc);
           } else if constexpr (ST == OT_G2D) {
-            v = g2d(from_left(c), c, from_right(c), tc, bc);
+            const float dxr = sq(c - from_right(c));
+            const float dyb = sq(c - bc);
+            v = c + rsq((((from_left(dxr) + eps4) + dxr) + dyt) + dyb);
+            dyt = dyb;
           } else if constexpr (ST == OT_P2D) {
             v = p2d(from_left(tc), tc, from_right(tc), from_left(c), c, from_right(c), from_left(bc), bc,
                     from_right(bc));
@@ -165,12 +181,19 @@ __device__ __forceinline__ void tile(const LaunchGeom& g, const FieldPtrs& f, co
         rk.sigma2 = g.coef[0] != 0.0f ? g.coef[0] : 0.0025f;  // SIGMA2 (rician-denoise-2d.cpp:159)
         rk.gamma = (g.coef[1] != 0.0f ? g.coef[1] : 0.065f) / rk.sigma2;
         rk.dt = g.coef[2] != 0.0f ? g.coef[2] : 5.0f;
-        // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]
+        // g on rows [2s-1, FR-2s], then u on rows [2s, FR-1-2s]. g's squared
+        // differences are shared as gradient-2d's: the point's +x and +y ones,
+        // the -x one from the lane to its left, the -y one from the row above,
+        // in g's order (right, left, bottom, top; rician-denoise-2d.cpp:271-281)
+        float dyb[FR];
+#pragma unroll
+        for (int i = 2 * s - 2; i <= FR - 2 * s; ++i) dyb[i] = sq(u[0][i] - u[0][i + 1]);
         float gg[FR];
 #pragma unroll
         for (int i = 2 * s - 1; i <= FR - 2 * s; ++i) {
           const float c = u[0][i];
-          gg[i] = mask(rician_g(c, from_left(c), from_right(c), u[0][i - 1], u[0][i + 1]), i);
+          const float dxr = sq(c - from_right(c));
+          gg[i] = mask(rsq((((1.0e-20f + dxr) + from_left(dxr)) + dyb[i]) + dyb[i - 1]), i);
         }
         float nu[FR];
 #pragma unroll
