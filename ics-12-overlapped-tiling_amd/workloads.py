@@ -27,7 +27,7 @@ WORKLOADS = {
                    tiled=dict(T=7, x=50, y=18, z=1, e=1), t1=dict(T=1, x=62, y=14, z=1, e=1),
                    bytes_pp=24, data="Ex = Ey = 0, Hz Gaussian pulse (amplitude 1, sigma 8) at the centre"),
     "gradient3d": dict(stencil="gradient-3d", n=(256, 256, 256), steps=64,
-                       tiled=dict(T=2, x=60, y=4, z=16, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),
+                       tiled=dict(T=2, x=60, y=8, z=14, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),
                        bytes_pp=8, data="sin(2 pi x/256) sin(2 pi y/256) sin(2 pi z/256) + U(-1e-3, 1e-3), seed 7"),
     "poisson3d-rhs": dict(stencil="poisson-3d-rhs", n=(256, 256, 256), steps=32,
                           tiled=dict(T=2, x=60, y=4, z=29, e=1), t1=dict(T=1, x=62, y=6, z=64, e=1),
