@@ -30,7 +30,7 @@ WORKLOADS = {
                        tiled=dict(T=2, x=60, y=8, z=14, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),
                        bytes_pp=8, data="sin(2 pi x/256) sin(2 pi y/256) sin(2 pi z/256) + U(-1e-3, 1e-3), seed 7"),
     "poisson3d-rhs": dict(stencil="poisson-3d-rhs", n=(256, 256, 256), steps=32,
-                          tiled=dict(T=2, x=60, y=4, z=29, e=1), t1=dict(T=1, x=62, y=6, z=64, e=1),
+                          tiled=dict(T=2, x=60, y=8, z=22, e=1), t1=dict(T=1, x=62, y=6, z=64, e=1),
                           bytes_pp=12, data="u0 = 0, f uniform [-1, 1) from seed 11"),
     # compute-bound (4 rsqrt and 5 divisions per point): the sweep's best tile is
     # a T=1 tile, so the time-tiled and T=1 lines coincide
