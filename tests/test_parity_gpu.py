@@ -252,6 +252,9 @@ FULL_SIZE = [
     # the deepest time tiles of the 256^3 sweep: the wave kernel at T=5 and the
     # LDS kernel at T=8, down to its smallest tile 8 x 8 x 4
     (otlib.J3D, (256, 256, 256), 32, (5, (54, 2, 20), 1), ((64, 4, 32), 4)),
+    # the sweep's best on some boxes: T=4 with 16 footprint rows, 1920 waves
+    # (issue balancing on; the T=5 tile above, 8320 waves, runs without it)
+    (otlib.J3D, (256, 256, 256), 32, (4, (56, 8, 22), 1), ((64, 4, 32), 4)),
     (otlib.J3D, (256, 256, 256), 32, (8, (16, 8, 16), 1), ((64, 4, 32), 4)),
     (otlib.J3D, (256, 256, 256), 32, (8, (8, 8, 4), 1), ((64, 4, 32), 4)),
     (otlib.G3D, (256, 256, 256), 16, (3, (58, 6, 20), 1), ((64, 4, 32), 4)),
