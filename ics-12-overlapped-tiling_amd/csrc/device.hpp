@@ -19,35 +19,29 @@ namespace dev {
 // the reference value of every rsqrt below.
 __device__ __forceinline__ float rsq_ieee(float x) { return 1.0f / __builtin_sqrtf(x); }
 
-// The same value from the hardware square root and reciprocal estimates,
-// for the arguments the stencils produce (x >= 1e-20: every rsqrt argument
-// is a sum with a positive epsilon of 1e-4, 1e-8 or 1e-20):
-//   s = RN(sqrt(x)): the v_sqrt_f32 estimate (within 1 ulp) moved by one ulp
-//     when the FMA residual x - s'*s of a neighbour s' shows the root lies
-//     beyond the midpoint between them;
-//   RN(1/s): the v_rcp_f32 estimate refined by one Newton step
-//     y' = y + y(1 - s y) and a Markstein correction y'' = y' + y'(1 - s y')
-//     (P. Markstein, "Computation of elementary functions on the IBM RISC
-//     System/6000 processor", IBM J. Res. Dev. 34(1), 1990), all residuals
-//     exact through FMA;
+// The same value from the hardware reciprocal-square-root and reciprocal
+// estimates, for the arguments the stencils produce (x >= 1e-20: every rsqrt
+// argument is a sum with a positive epsilon of 1e-4, 1e-8, 1e-9 or 1e-20):
+//   s = RN(sqrt(x)): from the v_rsq_f32 estimate y0 ~ 1/sqrt(x), the product
+//     s0 = x y0 corrected once by s = s0 + (y0/2)(x - s0^2), the residual
+//     exact through FMA (P. Markstein, "Computation of elementary functions on
+//     the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990);
+//   RN(1/s): the v_rcp_f32 estimate with one correction y' = y + y(1 - s y),
+//     again with the residual exact through FMA;
 //   x = +inf gives 0, NaN stays NaN.
-// About 16 VALU instructions against 26 for the IEEE expression, which also
-// guards denormal and zero inputs. ot_selftest_rsq checks the two bit for bit
-// on every float of [1e-20, +inf].
+// Neither correction is correctly rounded for every input in theory; both
+// are on this hardware for every float of [1e-20, +inf], which
+// ot_selftest_rsq checks bit for bit against the IEEE expression. 8 VALU
+// instructions and 2 transcendental ones, against 26 for the IEEE expression
+// (which also guards denormal and zero inputs). Starting the reciprocal from
+// y0 instead of v_rcp_f32 misses 196 inputs (s just below a power of two,
+// where the correction from below ends on a tie and rounds to even).
 __device__ __forceinline__ float rsq(float x) {
-  float s = __builtin_amdgcn_sqrtf(x);
-  const int si = __builtin_bit_cast(int, s);
-  const float sm = __builtin_bit_cast(float, si - 1);
-  const float sp = __builtin_bit_cast(float, si + 1);
-  const float rm = __builtin_fmaf(-sm, s, x);
-  const float rp = __builtin_fmaf(-sp, s, x);
-  s = (rm <= 0.0f) ? sm : s;
-  s = (rp > 0.0f) ? sp : s;
+  const float y0 = __builtin_amdgcn_rsqf(x);
+  const float s0 = x * y0;
+  const float s = __builtin_fmaf(__builtin_fmaf(-s0, s0, x), 0.5f * y0, s0);
   float y = __builtin_amdgcn_rcpf(s);
-  const float e = __builtin_fmaf(-s, y, 1.0f);
-  y = __builtin_fmaf(e, y, y);
-  const float r = __builtin_fmaf(-s, y, 1.0f);
-  y = __builtin_fmaf(r, y, y);
+  y = __builtin_fmaf(__builtin_fmaf(-s, y, 1.0f), y, y);
   return (x == __builtin_inff()) ? 0.0f : y;
 }
 __device__ __forceinline__ float sq(float x) { return x * x; }
