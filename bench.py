@@ -195,9 +195,9 @@ def _cpu_model():
 
 def batch_kernel_name(st, T, bx, by, e):
     """The wave-per-tile batch kernel runs tiles whose x footprint fills a
-    wavefront with 16, 32 or 64 footprint rows; others run the LDS kernel."""
+    wavefront with 16, 24, 32 or 64 footprint rows; others run the LDS kernel."""
     h = 2 * otlib.stencil_info(st).radius * T
-    wave = bx + h == 64 and e == 1 and by + h in (16, 32, 64)
+    wave = bx + h == 64 and e == 1 and by + h in (16, 24, 32, 64)
     return f"ot::k2w::k2w_batch<{otlib.STENCILS[st]}>" if wave else f"ot::k2d_batch<{otlib.STENCILS[st]}>"
 
 

@@ -313,6 +313,7 @@ ot_status launch_fr(int fr, const LaunchGeom& g, const FieldPtrs& f, void* strea
     if constexpr (V > 2 * H && kFields<ST> * V <= kW2RegBudget) return launch_t<ST, T, V>(g, f, stream, prepare); \
     break;
     FRC(16)
+    FRC(24)
     FRC(32)
     FRC(64)
 #undef FRC
@@ -347,6 +348,7 @@ BatchKernel pick_batch_fr(int fr) {
     if constexpr (V > 2 * H && kFields<ST> * V <= kW2RegBudget) return &k2w_batch<ST, T, V>; \
     break;
     FRB(16)
+    FRB(24)
     FRB(32)
     FRB(64)
 #undef FRB

@@ -86,7 +86,7 @@ int k2w_rows(int stencil, int T, const int32_t tile[3]) {
   const int fields = (stencil == OT_FDTD2D || stencil == OT_RICIAN2D || stencil == OT_TV2D) ? 3 : 1;
   if (tile[0] + 2 * R * T != 64) return 0;
   const int fr = tile[1] + 2 * R * T;
-  static const int kRows[] = {16, 32, 64};
+  static const int kRows[] = {16, 24, 32, 64};
   for (int r : kRows)
     if (r == fr && fr > 2 * R * T && fields * fr <= kW2RegBudget) return fr;
   return 0;

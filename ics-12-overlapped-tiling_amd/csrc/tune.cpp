@@ -47,7 +47,7 @@ void candidates(const ot_params& base, int dims, std::vector<Cand>& out) {
     // wave-per-tile kernel: x tile + 2*radius*T fills the 64 lanes
     const int R = kTraits[stencil].radius;
     for (int T = 1; T <= 8; ++T)
-      for (int fr : {16, 32, 64})
+      for (int fr : {16, 24, 32, 64})
         if (fr > 2 * R * T) out.push_back({T, 64 - 2 * R * T, fr - 2 * R * T, 1, 1});
     // row-streaming kernel: the same x tiles streaming chunks of 64..512 rows
     for (int T = 1; T <= (stencil == OT_RICIAN2D ? 5 : 8); ++T)

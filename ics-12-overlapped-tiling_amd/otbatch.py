@@ -126,7 +126,7 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
         (1, 2, 4)) if by % e == 0]
     for T in range(1, 9):
         x = 64 - 2 * R * T
-        for fr in (16, 32):
+        for fr in (16, 24, 32):
             if x > 0 and fr > 2 * R * T:
                 cfgs.append((T, x, fr - 2 * R * T, 1))
     out = []
@@ -138,7 +138,7 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
                 continue
             try:
                 fr = rows + 2 * R * T  # footprint rows of a wave tile
-                wave = bx + 2 * R * T == 64 and e == 1 and fr in (16, 32, 64) and 3 * fr <= 128
+                wave = bx + 2 * R * T == 64 and e == 1 and fr in (16, 24, 32, 64) and 3 * fr <= 128
                 out.append(dict(T=T, tile=[bx, rows], e=e, wave=int(wave),
                                 batched_ms=round(best_ms(bt, True, False, 3), 4),
                                 per_image_ms=round(min(best_ms(bt, False, True, 1), best_ms(bt, True, True, 1)), 4)))

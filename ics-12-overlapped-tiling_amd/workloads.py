@@ -24,7 +24,7 @@ WORKLOADS = {
                      tiled=dict(T=8, x=48, y=16, z=1, e=1), t1=dict(T=1, x=16, y=16, z=1, e=4),
                      bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
     "fdtd2d": dict(stencil="fdtd-2d", n=(2048, 2048, 1), steps=100,
-                   tiled=dict(T=7, x=50, y=18, z=1, e=1), t1=dict(T=1, x=62, y=14, z=1, e=1),
+                   tiled=dict(T=7, x=50, y=10, z=1, e=1), t1=dict(T=1, x=62, y=14, z=1, e=1),
                    bytes_pp=24, data="Ex = Ey = 0, Hz Gaussian pulse (amplitude 1, sigma 8) at the centre"),
     "gradient3d": dict(stencil="gradient-3d", n=(256, 256, 256), steps=64,
                        tiled=dict(T=2, x=60, y=8, z=14, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),

@@ -60,6 +60,15 @@ CASES_2D = [
     (otlib.RICIAN2D, (90, 61, 1), 7, 2, (56, 8, 1), 1),
     (otlib.RICIAN2D, (110, 70, 1), 9, 4, (48, 16, 1), 1),
     (otlib.RICIAN2D, (100, 50, 1), 7, 3, (52, 20, 1), 1),
+    # 24 footprint rows
+    (otlib.J2D, (100, 77, 1), 9, 4, (56, 16, 1), 1),
+    (otlib.P2D, (90, 61, 1), 7, 2, (60, 20, 1), 1),
+    (otlib.G2D, (90, 61, 1), 9, 3, (58, 18, 1), 1),
+    (otlib.TV2D, (90, 61, 1), 7, 5, (54, 14, 1), 1),
+    (otlib.FDTD2D, (130, 70, 1), 15, 7, (50, 10, 1), 1),
+    (otlib.FDTD2D, (90, 61, 1), 8, 8, (48, 8, 1), 1),
+    (otlib.RICIAN2D, (90, 61, 1), 7, 3, (52, 12, 1), 1),
+    (otlib.RICIAN2D, (90, 61, 1), 9, 5, (44, 4, 1), 1),
     # row-streaming kernels: the same x tiles, y tile = streamed chunk of >= 64 rows
     (otlib.J2D, (200, 150, 1), 9, 2, (60, 64, 1), 1),
     (otlib.J2D, (130, 230, 1), 19, 8, (48, 100, 1), 1),
@@ -263,6 +272,7 @@ FULL_SIZE = [
     (otlib.J2D, (1024, 1024, 1), 64, (4, (16, 16, 1), 1), ((64, 4, 1), 4)),
     # the 2D wave kernel at the bench's tiles (mask-free path on interior tiles)
     (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 18, 1), 1), ((64, 4, 1), 4)),
+    (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 10, 1), 1), ((64, 4, 1), 4)),
     (otlib.J2D, (1024, 1024, 1), 64, (8, (48, 16, 1), 1), ((64, 4, 1), 4)),
     # cs-3d plans only wave tiles, so its T=1 comparison is the bench's T=1 tile
     (otlib.CS3D, (256, 256, 256), 8, (2, (60, 8, 16), 1), ((62, 6, 10), 1)),

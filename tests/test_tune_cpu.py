@@ -126,7 +126,7 @@ def test_tune_entry_points_reject_bad_arguments(otlib):
 
 def test_space_holds_all_2d_kernel_families(otlib):
     """2D spaces carry the LDS tiles, the wave-per-tile kernel (x tile + 2rT = 64,
-    footprint rows 16/32/64) and the row-streaming kernel (chunks of >= 64 rows);
+    footprint rows 16/24/32/64) and the row-streaming kernel (chunks of >= 64 rows);
     Rician's streaming kernel stops at T = 5."""
     for name, R in (("jacobi-2d", 1), ("fdtd-2d", 1), ("rician-denoise-2d", 2)):
         sp = ottune.space(_p(name, (2048, 2048, 1), 24, 1, 16, 4, 1, 1))
@@ -137,7 +137,7 @@ def test_space_holds_all_2d_kernel_families(otlib):
         for q in sp:
             d = otlib.derive(q)
             is_wave = q.block_x + 2 * R * q.time_tile == 64 and (q.block_y >= 64 or q.block_y + 2 * R * q.time_tile
-                                                                in (16, 32, 64))
+                                                                in (16, 24, 32, 64))
             if is_wave and not (name == "rician-denoise-2d" and q.block_y >= 64 and q.time_tile > 5):
                 assert d.lds_bytes == 0 and d.threads_per_block == 256, (name, q.time_tile, q.block_y)
         if name == "rician-denoise-2d":
