@@ -21,7 +21,7 @@ WORKLOADS = {
                      tiled=dict(T=3, x=58, y=6, z=19, e=1), t1=dict(T=1, x=62, y=6, z=8, e=1),
                      bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
     "jacobi2d": dict(stencil="jacobi-2d", n=(1024, 1024, 1), steps=64,
-                     tiled=dict(T=8, x=48, y=16, z=1, e=1), t1=dict(T=1, x=16, y=16, z=1, e=4),
+                     tiled=dict(T=8, x=48, y=8, z=1, e=1), t1=dict(T=1, x=16, y=16, z=1, e=4),
                      bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
     "fdtd2d": dict(stencil="fdtd-2d", n=(2048, 2048, 1), steps=100,
                    tiled=dict(T=7, x=50, y=10, z=1, e=1), t1=dict(T=1, x=62, y=14, z=1, e=1),

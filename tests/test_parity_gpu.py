@@ -274,6 +274,7 @@ FULL_SIZE = [
     (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 18, 1), 1), ((64, 4, 1), 4)),
     (otlib.FDTD2D, (2048, 2048, 1), 100, (7, (50, 10, 1), 1), ((64, 4, 1), 4)),
     (otlib.J2D, (1024, 1024, 1), 64, (8, (48, 16, 1), 1), ((64, 4, 1), 4)),
+    (otlib.J2D, (1024, 1024, 1), 64, (8, (48, 8, 1), 1), ((64, 4, 1), 4)),
     # cs-3d plans only wave tiles, so its T=1 comparison is the bench's T=1 tile
     (otlib.CS3D, (256, 256, 256), 8, (2, (60, 8, 16), 1), ((62, 6, 10), 1)),
 ]
