@@ -8,8 +8,9 @@ clock records, and summarises the last launch: when the waves start and end
 (s_memrealtime, 100 MHz, one time base for the whole device), how long each
 spends in the pipeline fill, the steady-state blocks and the rest (s_memtime
 cycles, converted at the measured shader clock), how many waves each SIMD held,
-and how the lifetime of a wave depends on its start order on its SIMD (the
-hardware issues the oldest wave first unless the kernel balances progress).
+how the lifetime of a wave depends on its start order on its SIMD (the
+hardware issues the oldest wave first unless the kernel balances progress), how
+the dies (XCDs) compare, and which z-chunks hold the waves that end last.
 
   python3 ics-12-overlapped-tiling_amd/otclocks.py                 # T=3 58x6x19
   python3 ics-12-overlapped-tiling_amd/otclocks.py --no-balance    # oldest-first issue
@@ -99,6 +100,18 @@ def summarise(recs) -> str:
         "mean lifetime by start order on the SIMD: "
         + ", ".join(f"{o}: {life[order == o].mean():.2f} us" for o in range(order.max() + 1)),
     ]
+    lines.append("by XCD (XCC_ID): mean lifetime / last end, us; shader clock, MHz: "
+                 + ", ".join(f"{x}: {life[xcc == x].mean():.2f} / {end[xcc == x].max():.2f}; "
+                             f"{np.median((ck[xcc == x, 4] - ck[xcc == x, 0]) / np.maximum(life[xcc == x], 1e-3)):.0f}"
+                             for x in np.unique(xcc)))
+    # by z-chunk (the record's wave field): which chunks hold the waves that end last
+    bz = np.array([r.wave for r in recs], dtype=np.int64)
+    last = np.argsort(end, kind="stable")[-max(1, len(recs) // 20):]
+    lines.append("by z-chunk: mean lifetime / last end, us; waves among the last-ending 5 %:")
+    for z in np.unique(bz):
+        m = bz == z
+        lines.append(f"  chunk {z:3d}: {m.sum():5d} waves  {life[m].mean():6.2f} / {end[m].max():6.2f}"
+                     f"  {int(np.sum(bz[last] == z))}")
     return "\n".join(lines)
 
 
