@@ -14,6 +14,7 @@ the dies (XCDs) compare, and which z-chunks hold the waves that end last.
 
   python3 ics-12-overlapped-tiling_amd/otclocks.py                 # T=3 58x6x19
   python3 ics-12-overlapped-tiling_amd/otclocks.py --no-balance    # oldest-first issue
+  python3 ics-12-overlapped-tiling_amd/otclocks.py --offset-mb 19  # fields 19 MiB into their allocations
 """
 from __future__ import annotations
 
@@ -36,7 +37,17 @@ class ClockRecord(C.Structure):
                 ("realtime", C.c_uint64 * 2)]
 
 
-def records(n: int, T: int, bx: int, by: int, bz: int, variant: int, settle_s: float):
+class _At:
+    """A field at a byte offset inside a larger device allocation."""
+
+    def __init__(self, ptr: int):
+        self.ptr = ptr
+
+
+def records(n: int, T: int, bx: int, by: int, bz: int, variant: int, settle_s: float, offset_bytes: int = 0):
+    """Clock records of the last launch; offset_bytes places both fields that
+    far into allocations of field size + offset (to see whether timing follows
+    the addresses)."""
     w = workloads.WORKLOADS["jacobi3d"]
     cfg = dict(T=T, x=bx, y=by, z=bz, e=1)
     p = workloads.plan_params(w, cfg, (n, n, n), 10 * T)
@@ -44,8 +55,15 @@ def records(n: int, T: int, bx: int, by: int, bz: int, variant: int, settle_s: f
     plan = otlib.Plan(p)
     d = plan.derived
     rw, _ = workloads.make_inputs(w, d)
-    A = [otlib.DeviceBuffer.from_array(x) for x in rw]
-    B = [otlib.DeviceBuffer.from_array(x) for x in rw]
+    allocs = []
+    A, B = [], []
+    for x in rw:
+        x = np.ascontiguousarray(x)
+        for side in (A, B):
+            buf = otlib.DeviceBuffer(x.nbytes + offset_bytes)
+            allocs.append(buf)
+            side.append(_At(buf.ptr + offset_bytes))
+            otlib.check(otlib.lib().ot_memcpy_h2d(side[-1].ptr, x.ctypes.data, x.nbytes))
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < settle_s:
         for _ in range(8):
@@ -57,7 +75,7 @@ def records(n: int, T: int, bx: int, by: int, bz: int, variant: int, settle_s: f
     recs = (ClockRecord * cnt.value)()
     otlib.check(otlib.lib().ot_plan_clocks(plan.handle, recs, cnt.value, C.byref(cnt)))
     plan.close()
-    for b in A + B:
+    for b in allocs:
         b.free()
     return [r for r in recs if r.valid]
 
@@ -124,9 +142,11 @@ def main():
     ap.add_argument("--bz", type=int, default=19)
     ap.add_argument("--no-balance", action="store_true", help="oldest-first issue (OT_VARIANT_NO_BALANCE)")
     ap.add_argument("--settle", type=float, default=0.3)
+    ap.add_argument("--offset-mb", type=int, default=0,
+                    help="place both fields this many MiB into larger allocations")
     a = ap.parse_args()
     variant = otlib.VARIANT_NO_BALANCE if a.no_balance else 0
-    recs = records(a.n, a.T, a.bx, a.by, a.bz, variant, a.settle)
+    recs = records(a.n, a.T, a.bx, a.by, a.bz, variant, a.settle, a.offset_mb << 20)
     print(f"jacobi-3d {a.n}^3, T={a.T} tile {a.bx}x{a.by}x{a.bz}, "
           f"{'oldest-first issue' if a.no_balance else 'default issue policy'}; last launch of a "
           f"{10 * a.T}-step solve")
