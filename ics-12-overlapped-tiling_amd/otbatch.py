@@ -97,9 +97,10 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
     work-queue launch (hipGraph replay, best of 3) and one per-grid launch per
     image and time tile (faster of eager and graph). Candidates are LDS tiles
     (T 1..6, 9 block shapes, 1/2/4 elements per thread) and the tiles that fill
-    a wavefront (x tile + 2rT = 64, footprint rows 16 or 32). Returns one dict
+    a wavefront (x tile + 2rT = 64, footprint rows 16, 24 or 32). Returns one dict
     per configuration that builds."""
     import itertools
+    import sys
     import time
 
     st = otlib.stencil_id(stencil)
@@ -142,6 +143,8 @@ def sweep(stencil: str, n_images: int = 256, iterations: int = 50, seed: int = 7
                 out.append(dict(T=T, tile=[bx, rows], e=e, wave=int(wave),
                                 batched_ms=round(best_ms(bt, True, False, 3), 4),
                                 per_image_ms=round(min(best_ms(bt, False, True, 1), best_ms(bt, True, True, 1)), 4)))
+                print(f"{name} T={T} {bx}x{rows} e{e}: batched {out[-1]['batched_ms']} ms, "
+                      f"per image {out[-1]['per_image_ms']} ms", file=sys.stderr, flush=True)
             finally:
                 bt.close()
     finally:
