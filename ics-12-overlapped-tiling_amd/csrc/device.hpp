@@ -46,6 +46,18 @@ __device__ __forceinline__ float rsq(float x) {
 }
 __device__ __forceinline__ float sq(float x) { return x * x; }
 
+// x * m under the DX9 rule 0 * anything = +0 (v_mul_legacy_f32, one VALU
+// instruction like v_mul_f32; clang exposes no builtin for it). For any m != 0
+// it is the IEEE product. It lets a Dirichlet mask ride on an update's final
+// multiply (m = 0 outside the domain) and still give exactly the +0 that the
+// reference keeps on its ring, also when the masked sum is negative, infinite
+// or NaN.
+__device__ __forceinline__ float mul_legacy(float x, float m) {
+  float r;
+  asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(m));
+  return r;
+}
+
 // Wavefront x-neighbours for the wave-per-tile kernels (lanes = consecutive x
 // columns): DPP wave_shr:1 / wave_shl:1 with bound_ctrl, folded by the compiler
 // into the consuming VALU instruction.

@@ -124,11 +124,10 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, in
   }
   // jacobi-3d: the Dirichlet mask rides on the update's multiply. A point in
   // the domain takes sum * 0.143f, the very product of the update; a point
-  // outside it takes sum * 0, a zero for every finite sum, where the other
-  // stencils AND their result with the mask. This saves one VALU instruction
-  // of the eight per evaluation. (A zero's sign can differ from the +0 of the
-  // reference's ring only when the sum is negative; it then changes no
-  // neighbour's bits unless that neighbour's seven terms are all zeros.)
+  // outside it takes sum * 0 under the DX9 rule (mul_legacy), +0 whatever the
+  // sum (negative, infinite or NaN), as on the reference's ring, where the
+  // other stencils AND their result with the mask. This saves one VALU
+  // instruction of the eight per evaluation.
   constexpr bool kMulMask = ST == OT_J3D;
   float cmul[kMulMask ? FR : 1];
 #pragma unroll
@@ -347,7 +346,7 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, in
           const float ym = q[s - 1][Q1][i - 1];
           const float yp = q[s - 1][Q1][i + 1];
           const float xm = from_left(c), xp = from_right(c);
-          if constexpr (kMulMask) v = j3d_sum(xm, xp, ym, yp, zm, zp, c) * cmul[i];
+          if constexpr (kMulMask) v = mul_legacy(j3d_sum(xm, xp, ym, yp, zm, zp, c), cmul[i]);
           else if constexpr (ST == OT_G3D) v = g3d(xm, xp, ym, yp, zm, zp, c);
           else if constexpr (ST == OT_CS3D)
             // the rsqrt of G at the centre, at z-1 (the previous plane's

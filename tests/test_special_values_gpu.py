@@ -4,7 +4,11 @@ smallest, 2^-149), signed zeros and huge finite values (1e18, whose squares in
 the gradient stencils reach 1e36 and whose Rician correction overflows to inf
 and then NaN); and "tiny", the same fields scaled by 1e-38 (most values
 denormal, so the stencils' arithmetic runs in the denormal range) and
-sprinkled with zeros and the smallest denormals. The engine
+sprinkled with zeros and the smallest denormals; and "nonfinite", sprinkled
+with infinities, NaN and values near the largest float, whose sums overflow.
+Points outside the domain must stay +0 whatever a masked update computes
+there (infinite, NaN or negative sums included), as on the reference's ring.
+The engine
 must give the same bits at every point where the oracle's value is not NaN,
 and NaN exactly where the oracle's is (NaN payloads differ between the CPU,
 whose invalid operations give a negative quiet NaN, and the GPU).
@@ -19,9 +23,11 @@ from helpers import domain_shape, random_inputs, run_gpu
 pytestmark = pytest.mark.gpu
 
 S = otlib
+# name: (scale of the field, values sprinkled in, fraction of points replaced)
 SETS = {
-    "wide": (1.0, np.array([0.0, -0.0, 1e-40, -1e-40, 2.0 ** -149, -(2.0 ** -149), 3e-39, 1e18, -1e18], np.float32)),
-    "tiny": (1e-38, np.array([0.0, -0.0, 2.0 ** -149, -(2.0 ** -149), 2.0 ** -140], np.float32)),
+    "wide": (1.0, np.array([0.0, -0.0, 1e-40, -1e-40, 2.0 ** -149, -(2.0 ** -149), 3e-39, 1e18, -1e18], np.float32), 0.03),
+    "tiny": (1e-38, np.array([0.0, -0.0, 2.0 ** -149, -(2.0 ** -149), 2.0 ** -140], np.float32), 0.03),
+    "nonfinite": (1.0, np.array([np.inf, -np.inf, np.nan, -3e38, 3e38], np.float32), 0.001),
 }
 # (stencil, (nx, ny, nz), steps, T, block, elements per thread): one case per kernel family and stencil
 CASES = [
@@ -52,7 +58,7 @@ def _id(c):
     return f"{S.STENCILS[st]}-n{'x'.join(map(str, n3))}-T{T}-b{'x'.join(map(str, block))}-e{e}"
 
 
-def _sprinkle(a: np.ndarray, rng, scale, specials, frac=0.03) -> np.ndarray:
+def _sprinkle(a: np.ndarray, rng, scale, specials, frac) -> np.ndarray:
     a = (a * np.float32(scale)).astype(np.float32)
     flat = a.reshape(-1)
     idx = rng.choice(flat.size, size=max(1, int(frac * flat.size)), replace=False)
@@ -76,8 +82,8 @@ def test_special_values_match_oracle(otlib, case, which):
     d = otlib.derive(p)
     rng = np.random.default_rng(1000 + st * 10 + T)
     rw, ro = random_inputs(st, domain_shape(d), seed=st * 100 + T)
-    scale, specials = SETS[which]
-    rw = [_sprinkle(x, rng, scale, specials) for x in rw]
+    scale, specials, frac = SETS[which]
+    rw = [_sprinkle(x, rng, scale, specials, frac) for x in rw]
     print(f"case {_id(case)} {which} steps={steps}", flush=True)
     gpu, _ = run_gpu(p, rw, ro)
     ref = oracle.run(st, [x.copy() for x in rw], [x for x in ro], steps=steps, coef=list(p.coef), variant=p.variant)
