@@ -46,12 +46,14 @@ __device__ __forceinline__ float rsq(float x) {
 }
 __device__ __forceinline__ float sq(float x) { return x * x; }
 
-// x * m under the DX9 rule 0 * anything = +0 (v_mul_legacy_f32, one VALU
-// instruction like v_mul_f32; clang exposes no builtin for it). For any m != 0
-// it is the IEEE product. It lets a Dirichlet mask ride on an update's final
-// multiply (m = 0 outside the domain) and still give exactly the +0 that the
-// reference keeps on its ring, also when the masked sum is negative, infinite
-// or NaN.
+// x * m under the DX9 rule (v_mul_legacy_f32, one VALU instruction like
+// v_mul_f32; clang exposes no builtin for it): a zero operand of either sign
+// gives +0 whatever the other one is (infinite and NaN included), and two
+// nonzero operands give the IEEE product, denormals kept. It lets a Dirichlet
+// mask ride on an update's final multiply (m = 0 outside the domain) and still
+// give exactly the +0 that the reference keeps on its ring, also when the
+// masked sum is negative, infinite or NaN. Inside (m != 0) it differs from the
+// IEEE product only for x = -0, which it turns into +0.
 __device__ __forceinline__ float mul_legacy(float x, float m) {
   float r;
   asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(m));

@@ -127,7 +127,9 @@ __global__ void __launch_bounds__(kWaves * 64) k3w(LaunchGeom g, FieldPtrs f, in
   // outside it takes sum * 0 under the DX9 rule (mul_legacy), +0 whatever the
   // sum (negative, infinite or NaN), as on the reference's ring, where the
   // other stencils AND their result with the mask. This saves one VALU
-  // instruction of the eight per evaluation.
+  // instruction of the eight per evaluation. (The one input on which the
+  // rule departs from the reference: a point whose seven terms are all -0.0
+  // has the sum -0 and gets +0 where the reference's product is -0.)
   constexpr bool kMulMask = ST == OT_J3D;
   float cmul[kMulMask ? FR : 1];
 #pragma unroll
