@@ -35,7 +35,7 @@ WORKLOADS = {
     # compute-bound (4 rsqrt and 5 divisions per point): the sweep's best tile is
     # a T=1 tile, so the time-tiled and T=1 lines coincide
     "cs3d": dict(stencil="cs-3d", n=(256, 256, 256), steps=64,
-                 tiled=dict(T=1, x=62, y=6, z=14, e=1), t1=dict(T=1, x=62, y=6, z=14, e=1),
+                 tiled=dict(T=1, x=62, y=6, z=19, e=1), t1=dict(T=1, x=62, y=6, z=19, e=1),
                  bytes_pp=8, data="uniform [0,1) interior from glibc srand(123456), Dirichlet-0 boundary"),
 }
 
